@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X diffusion engine; prints ONE JSON line.
+
+Default workload: the 512^3 FP32 grid, cosine-product initial field and the
+reference's coefficients, advanced by the plain drop-in kernel (one step per
+launch, EXACT arithmetic, bitwise equal to the reference CPU path).  A "step"
+is one diffusion time step over the whole grid.  The field is resident in HBM
+before the timed region; the timed region is one synchronous d3h_field_run of
+exactly --steps steps (the library records HIP events on its own stream around
+the launches and synchronises on the last one, so the region is bracketed by
+device synchronisation on both sides).  value = Gcell-updates/s from the host
+wall clock of that call.
+
+roofline: algorithmic bytes per launch (2 x storage bytes x cells, one read
+and one write per cell, as the reference's GetThroughput) / average launch
+duration from those events, against the 8 TB/s HBM peak.  traffic is read
+from profiles/traffic_<workload>.json (rocprofv3 FETCH_SIZE / WRITE_SIZE),
+when present.
+
+cpu_baseline: the reference's OpenMP RunKernel (oracle/_ref/libref.so, built
+from the reference sources) or, when that library is absent, the oracle's C
+restatement, timed on this host's cores on a bounded sample of the same grid.
+
+Only one GPU is used: --gpus N > 1 exits with a message.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+WORKLOADS = {
+    # name: (nx, ny, nz, dtype, depth, default steps)
+    "plain512_f32": (512, 512, 512, "f32", 1, 1000),
+    "plain512_f64": (512, 512, 512, "f64", 1, 1000),
+    "tb768_f32": (768, 768, 768, "f32", 4, 2000),
+}
+
+
+def cpu_baseline(nx, ny, nz, steps_sample):
+    """Time the reference's OpenMP RunKernel (or the oracle port) on the host."""
+    import numpy as np
+    gomp = ctypes.CDLL("libgomp.so.1")
+    gomp.omp_get_max_threads.restype = ctypes.c_int
+    threads = gomp.omp_get_max_threads()
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    if os.path.exists(ref):
+        L = ctypes.CDLL(ref)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.ref_run.argtypes = [ctypes.c_int] * 5 + [fp, fp, fp, fp, ctypes.POINTER(ctypes.c_double)]
+        sec = ctypes.c_double()
+        rc = L.ref_run(1, nx, ny, nz, steps_sample, None, None, None, None, ctypes.byref(sec))
+        if rc != 0:
+            raise RuntimeError("ref_run failed")
+        seconds, kind = sec.value, "reference"
+    else:
+        import oracle
+        k = oracle.constants_f32(nx, ny, nz)
+        f = oracle.initialize_f32(nx, ny, nz, k)
+        g = np.empty_like(f)
+        c = np.ascontiguousarray(k[8:15])
+        L = oracle.lib()
+        t0 = time.perf_counter()
+        L.oracle_run_f32(f.ctypes.data_as(oracle._f), g.ctypes.data_as(oracle._f), nx, ny, nz,
+                         c.ctypes.data_as(oracle._f), steps_sample)
+        seconds, kind = time.perf_counter() - t0, "port"
+    cells = nx * ny * nz * steps_sample
+    return {"value": cells / seconds / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "kind": kind,
+            "sample": "%dx%dx%d FP32, %d steps, OpenMP RunKernel wall clock (%.2f s)"
+                      % (nx, ny, nz, steps_sample, seconds)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="plain512_f32", choices=sorted(WORKLOADS))
+    ap.add_argument("--arith", default="exact", choices=["exact", "fma"])
+    ap.add_argument("--depth", type=int, default=None, help="temporal depth (default per workload)")
+    ap.add_argument("--graph", type=int, default=1, help="replay a captured hipGraph (1) or launch")
+    ap.add_argument("--tune", type=int, default=1, help="autotune the tile before timing")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU (this build has no multi-GPU path); "
+              "got --gpus %d" % args.gpus, file=sys.stderr)
+        return 2
+
+    import benchmarks_amd as d3
+    nx, ny, nz, dname, depth, dsteps = WORKLOADS[args.workload]
+    steps = args.steps if args.steps is not None else dsteps
+    depth = args.depth if args.depth is not None else depth
+    dtype = {"f32": d3.F32, "f64": d3.F64}[dname]
+    arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
+    esize = 4 if dtype == d3.F32 else 8
+
+    ctx = d3.Context(0)
+    info = ctx.info()
+    print("device: %d CUs, %.1f GB HBM" % (info["num_cus"], info["total_bytes"] / 1e9),
+          file=sys.stderr, flush=True)
+    probe = ctx.bandwidth_probe(1 << 28, d3.F32, reps=10)
+    print("copy probe: %.1f GB/s" % probe["gbps"], file=sys.stderr, flush=True)
+    tuned = None
+    if args.tune:
+        tuned = ctx.autotune(nx, ny, nz, dtype, depth, arith)
+        print("tuned:", tuned, file=sys.stderr, flush=True)
+
+    if dtype == d3.F32:
+        k = d3.constants_f32(nx, ny, nz)
+        f0 = d3.initialize_f32(nx, ny, nz, k)
+    else:
+        k = d3.constants_f64(nx, ny, nz)
+        f0 = d3.initialize_f64(nx, ny, nz, k)
+    coef = d3.coefficients(k)
+    field = d3.Field(ctx, nx, ny, nz, dtype)
+    field.upload(f0)
+    del f0
+    if args.warmup > 0:
+        field.run(args.warmup, coef, depth=depth, arith=arith, graph=False)
+    # re-initialise after the warm-up (diffusion/diffusion.h:126-131)
+    field.upload(d3.initialize_f32(nx, ny, nz, k) if dtype == d3.F32 else
+                 d3.initialize_f64(nx, ny, nz, k))
+    if args.graph:
+        # capture once (untimed), restore the input, then time a replay
+        field.run(steps, coef, depth=depth, arith=arith, graph=True)
+        field.upload(d3.initialize_f32(nx, ny, nz, k) if dtype == d3.F32 else
+                     d3.initialize_f64(nx, ny, nz, k))
+    t0 = time.perf_counter()
+    tm = field.run(steps, coef, depth=depth, arith=arith, graph=bool(args.graph))
+    wall = time.perf_counter() - t0
+    print("timing:", tm, file=sys.stderr, flush=True)
+
+    cells = nx * ny * nz
+    gcells = cells * steps / wall / 1e9
+    err = field.error_analytical(float(k[0]), float(steps) * float(k[7]),
+                                 [float(v) for v in k[4:7]], [float(v) for v in k[1:4]])
+    field.close()
+
+    bytes_per_launch = 2.0 * esize * cells          # one read + one write per cell
+    avg_launch_s = tm["avg_launch_ms"] * 1e-3
+    steps_per_launch = tm["temporal_depth"] if tm["launches"] == steps // tm["temporal_depth"] else 1
+    # for fused launches the algorithmic bytes are those of the single step
+    # sequence they replace (effective bandwidth)
+    achieved = bytes_per_launch * steps_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(nx if dtype == d3.F32 else nx, ny, nz, args.cpu_steps)
+
+    line = {
+        "metric": "Gcell-updates/s + %HBM roofline at 512^3",
+        "value": round(gcells, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dname,
+        "data": "synthetic (analytical cosine-product field, reference coefficients)",
+        "config": {"workload": "%dx%dx%d %s 7-point diffusion, %d steps, depth %d, %s arithmetic"
+                               % (nx, ny, nz, dname.upper(), steps, depth, args.arith),
+                   "grid": [nx, ny, nz], "temporal_depth": depth, "arith": args.arith,
+                   "graph": bool(args.graph), "tile": tm["tile"], "zchunks": tm["zchunks"],
+                   "blocks": tm["blocks"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+        "kernel_ms": round(tm["kernel_ms"], 3),
+        "avg_launch_ms": round(tm["avg_launch_ms"], 5),
+        "copy_probe_gbps": round(probe["gbps"], 1),
+        "error_vs_analytical_fp64": {"rms": err["rms"], "max_abs": err["max_abs"]},
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,51 @@
+// Registry of compiled stencil kernels (one entry per dtype / lane vector /
+// temporal depth / tile shape / arithmetic mode).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "stencil.hpp"
+
+namespace d3h {
+
+typedef void (*LaunchFn)(const void *src, void *dst, const Geometry &g, const double *coef,
+                         unsigned blocks, hipStream_t stream);
+
+struct KernelEntry {
+  int dtype;  // StoreKind
+  int vec;    // cells per lane vector
+  int T;      // fused steps per launch
+  int G;      // lane vectors per row (groups)
+  int RPW;    // rows per wave
+  int NW;     // waves per workgroup
+  int fma;    // 0 = exact, 1 = fma chain
+  const void *func;
+  LaunchFn launch;
+  int tile_x() const { return 64 * G * vec; }
+  int tile_y() const { return NW * RPW; }
+};
+
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+void launch_stencil(const void *src, void *dst, const Geometry &g, const double *c, unsigned blocks,
+                    hipStream_t stream) {
+  using C = typename Traits<S>::C;
+  using ST = typename Traits<S>::T;
+  Coef<C> k{(C)c[0], (C)c[1], (C)c[2], (C)c[3], (C)c[4], (C)c[5], (C)c[6]};
+  hipLaunchKernelGGL((stencil_kernel<S, VEC, T, G, RPW, NW, FMA>), dim3(blocks), dim3(64 * NW), 0,
+                     stream, (const ST *)src, (ST *)dst, g, k);
+}
+
+#define D3H_ENTRY(S, VEC, T, G, RPW, NW, FMA)                                               \
+  KernelEntry {                                                                             \
+    S, VEC, T, G, RPW, NW, FMA, (const void *)&stencil_kernel<S, VEC, T, G, RPW, NW, FMA>,  \
+        &launch_stencil<S, VEC, T, G, RPW, NW, FMA>                                         \
+  }
+
+// Each kernels_*.hip file defines one table.
+const KernelEntry *kernels_f32_plain(int *n);
+const KernelEntry *kernels_f32_tb(int *n);
+const KernelEntry *kernels_f64(int *n);
+const KernelEntry *kernels_bf16(int *n);
+const KernelEntry *kernels_f16(int *n);
+
+}  // namespace d3h

@@ -1,0 +1,922 @@
+// Runtime of the diffusion engine: contexts, device-resident fields, the step
+// loop (direct launches or a captured hipGraph), FP64 error norms, dtype
+// conversion, the HBM probe, the tile table and autotuner, and the
+// Physis-shaped drop-in entry points.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "runtime.hpp"
+
+namespace d3h {
+
+static thread_local std::string t_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+
+size_t dtype_bytes(int dtype) {
+  switch (dtype) {
+    case D3H_F32: return 4;
+    case D3H_F64: return 8;
+    case D3H_BF16:
+    case D3H_F16: return 2;
+    default: return 0;
+  }
+}
+bool dtype_valid(int dtype) { return dtype >= D3H_F32 && dtype <= D3H_F16; }
+int pitch_elems(int nx, int dtype) {
+  const int v = 16 / (int)dtype_bytes(dtype);
+  return (nx + v - 1) / v * v;
+}
+
+const std::vector<KernelEntry> &registry() {
+  static const std::vector<KernelEntry> all = [] {
+    std::vector<KernelEntry> v;
+    const KernelEntry *(*tables[])(int *) = {kernels_f32_plain, kernels_f32_tb, kernels_f64,
+                                              kernels_bf16, kernels_f16};
+    for (auto fn : tables) {
+      int n = 0;
+      const KernelEntry *t = fn(&n);
+      v.insert(v.end(), t, t + n);
+    }
+    return v;
+  }();
+  return all;
+}
+
+const KernelEntry *default_entry(int dtype, int T, int fma) {
+  for (const auto &e : registry())
+    if (e.dtype == dtype && e.T == T && e.fma == fma) return &e;
+  return nullptr;
+}
+
+int occupancy_of(const KernelEntry &e) {
+  static std::mutex mu;
+  static std::map<const void *, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(e.func);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, e.func, 64 * e.NW, 0) != hipSuccess) nb = 0;
+  cache[e.func] = nb;
+  return nb;
+}
+
+int make_geometry(const KernelEntry &e, int nx, int ny, int nz, int pitch, int zchunks,
+                  int num_cus, int occupancy, Geometry *g, unsigned *blocks) {
+  const int TX = e.tile_x(), TY = e.tile_y(), T = e.T;
+  Geometry q{};
+  q.nx = nx; q.ny = ny; q.nz = nz; q.pitch = pitch;
+  q.plane = (long long)pitch * ny;
+  if (nx <= TX) {
+    q.ntx = 1; q.ox = nx; q.hx = 0;
+  } else {
+    q.hx = (T + e.vec - 1) / e.vec * e.vec;
+    q.ox = TX - 2 * q.hx;
+    if (q.ox <= 0) return fail(D3H_ERR_UNSUPPORTED, "tile %d wide cannot hold depth %d", TX, T);
+    q.ntx = (nx + q.ox - 1) / q.ox;
+  }
+  if (ny <= TY) {
+    q.nty = 1; q.oy = ny;
+  } else {
+    q.oy = TY - 2 * T;
+    if (q.oy <= 0) return fail(D3H_ERR_UNSUPPORTED, "tile %d rows cannot hold depth %d", TY, T);
+    q.nty = (ny + q.oy - 1) / q.oy;
+  }
+  const long long tiles = (long long)q.ntx * q.nty;
+  int nzc = zchunks;
+  if (nzc <= 0) {
+    // One full wave of resident workgroups, but keep chunks at least
+    // max(4T, 8) planes deep so the 2T halo planes stay a small overhead.
+    const long long slots = (long long)std::max(occupancy, 1) * std::max(num_cus, 1);
+    nzc = (int)std::max(1LL, slots / tiles);
+    nzc = std::min(nzc, std::max(1, nz / std::max(4 * T, 8)));
+  }
+  nzc = std::max(1, std::min(nzc, nz));
+  q.oz = (nz + nzc - 1) / nzc;
+  q.nzc = (nz + q.oz - 1) / q.oz;
+  const long long nb = tiles * q.nzc;
+  if (nb <= 0 || nb > INT_MAX) return fail(D3H_ERR_UNSUPPORTED, "grid too large");
+  *g = q;
+  *blocks = (unsigned)nb;
+  return D3H_OK;
+}
+
+static int ilog2ceil(int v) {
+  int r = 0;
+  while ((1 << r) < v) ++r;
+  return r;
+}
+
+TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz) {
+  return TuneKey{dtype, T, fma, ilog2ceil(nx), ilog2ceil(ny), ilog2ceil(nz)};
+}
+
+static const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW) {
+  for (const auto &e : registry())
+    if (e.dtype == dtype && e.T == T && e.fma == fma && e.vec == vec && e.G == G && e.RPW == RPW &&
+        e.NW == NW)
+      return &e;
+  return nullptr;
+}
+
+int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
+                  const KernelEntry **entry, int *zchunks) {
+  const int T = cfg.temporal_depth <= 0 ? 1 : cfg.temporal_depth;
+  const int fma = cfg.arith == D3H_ARITH_FMA ? 1 : 0;
+  if (cfg.arith != D3H_ARITH_EXACT && cfg.arith != D3H_ARITH_FMA)
+    return fail(D3H_ERR_INVALID, "unknown arith mode %d", cfg.arith);
+  const auto &reg = registry();
+  if (cfg.tile >= 0) {
+    if (cfg.tile >= (int)reg.size()) return fail(D3H_ERR_INVALID, "tile index %d out of range", cfg.tile);
+    const KernelEntry &e = reg[cfg.tile];
+    if (e.dtype != dtype || e.T != T || e.fma != fma)
+      return fail(D3H_ERR_INVALID, "tile %d is dtype %d depth %d fma %d, run wants %d/%d/%d", cfg.tile,
+                  e.dtype, e.T, e.fma, dtype, T, fma);
+    *entry = &e;
+    *zchunks = cfg.zchunks;
+    return D3H_OK;
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz));
+    if (it != ctx->tuned.end()) {
+      const KernelEntry *e = find_entry(dtype, T, fma, it->second.vec, it->second.G, it->second.RPW,
+                                        it->second.NW);
+      if (e) {
+        *entry = e;
+        *zchunks = cfg.zchunks > 0 ? cfg.zchunks : it->second.zchunks;
+        return D3H_OK;
+      }
+    }
+  }
+  const KernelEntry *e = default_entry(dtype, T, fma);
+  if (!e) return fail(D3H_ERR_UNSUPPORTED, "no kernel for dtype %d depth %d fma %d", dtype, T, fma);
+  *entry = e;
+  *zchunks = cfg.zchunks;
+  return D3H_OK;
+}
+
+int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEntry &e, int zchunks,
+                  hipStream_t stream, int *launches, unsigned *blocks_out) {
+  const int T = e.T;
+  const int nfull = count / T, rem = count % T;
+  Geometry g{}, g1{};
+  unsigned nb = 0, nb1 = 0;
+  D3H_TRY(make_geometry(e, f->nx, f->ny, f->nz, f->pitch, zchunks, f->ctx->num_cus, occupancy_of(e),
+                        &g, &nb));
+  const KernelEntry *plain = nullptr;
+  if (rem) {
+    plain = default_entry(f->dtype, 1, e.fma);
+    if (!plain) return fail(D3H_ERR_UNSUPPORTED, "no plain kernel for dtype %d", f->dtype);
+    D3H_TRY(make_geometry(*plain, f->nx, f->ny, f->nz, f->pitch, 0, f->ctx->num_cus,
+                          occupancy_of(*plain), &g1, &nb1));
+  }
+  int n = 0;
+  for (int i = 0; i < nfull; ++i, ++n) {
+    e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g, coef, nb, stream);
+    f->cur ^= 1;
+  }
+  for (int i = 0; i < rem; ++i, ++n) {
+    plain->launch(f->buf[f->cur], f->buf[f->cur ^ 1], g1, coef, nb1, stream);
+    f->cur ^= 1;
+  }
+  D3H_HIP(hipGetLastError());
+  if (launches) *launches = n;
+  if (blocks_out) *blocks_out = nb;
+  return D3H_OK;
+}
+
+// ---- conversion ------------------------------------------------------------
+__device__ __forceinline__ double load_elem(const void *p, int dtype, size_t i) {
+  switch (dtype) {
+    case D3H_F32: return (double)static_cast<const float *>(p)[i];
+    case D3H_F64: return static_cast<const double *>(p)[i];
+    case D3H_BF16: return (double)__uint_as_float((uint32_t) static_cast<const uint16_t *>(p)[i] << 16);
+    default: return (double)f16_bits_to_f32(static_cast<const uint16_t *>(p)[i]);
+  }
+}
+__device__ __forceinline__ void store_elem(void *p, int dtype, size_t i, double v) {
+  switch (dtype) {
+    case D3H_F32: static_cast<float *>(p)[i] = (float)v; break;
+    case D3H_F64: static_cast<double *>(p)[i] = v; break;
+    case D3H_BF16: static_cast<uint16_t *>(p)[i] = (uint16_t)to_bf16((float)v); break;
+    default: static_cast<uint16_t *>(p)[i] = (uint16_t)to_f16((float)v); break;
+  }
+}
+
+__global__ void convert_kernel(const void *src, int sdt, int sp, void *dst, int ddt, int dp, int nx,
+                               long long rows) {
+  const long long n = rows * nx;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / nx, x = i - row * nx;
+    store_elem(dst, ddt, (size_t)(row * dp + x), load_elem(src, sdt, (size_t)(row * sp + x)));
+  }
+}
+
+int launch_convert(const void *src, int src_dtype, int src_pitch, void *dst, int dst_dtype,
+                   int dst_pitch, int nx, int ny, int nz, hipStream_t stream) {
+  const long long rows = (long long)ny * nz;
+  const long long n = rows * nx;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(convert_kernel, dim3(blocks), dim3(256), 0, stream, src, src_dtype,
+                     src_pitch ? src_pitch : nx, dst, dst_dtype, dst_pitch ? dst_pitch : nx, nx, rows);
+  D3H_HIP(hipGetLastError());
+  return D3H_OK;
+}
+
+// ---- FP64 reductions --------------------------------------------------------
+constexpr int kRedThreads = 256;
+
+template <int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], double *part, bool last_is_max) {
+  __shared__ double sh[K][kRedThreads];
+#pragma unroll
+  for (int k = 0; k < K; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double o = sh[k][threadIdx.x + s];
+        if (last_is_max && k == K - 1) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], o);
+        else sh[k][threadIdx.x] += o;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[blockIdx.x * K + k] = sh[k][0];
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+compare_kernel(const void *a, int adt, int ap, const void *b, int bdt, int bp, int nx, long long rows,
+               double *part) {
+  double v[3] = {0.0, 0.0, 0.0};  // sum d^2, sum b^2, max |d|
+  const long long n = rows * nx;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / nx, x = i - row * nx;
+    const double av = load_elem(a, adt, (size_t)(row * ap + x));
+    const double bv = load_elem(b, bdt, (size_t)(row * bp + x));
+    const double d = av - bv;
+    v[0] += d * d;
+    v[1] += bv * bv;
+    v[2] = fmax(v[2], fabs(d));
+  }
+  block_reduce<3>(v, part, true);
+}
+
+// fx[i] = 1 - ax cos(kx x_i) etc. are precomputed per axis in FP64 on the host.
+__global__ void __launch_bounds__(kRedThreads)
+analytical_kernel(const void *f, int dt, int p, int nx, int ny, long long rows, const double *tab,
+                  double *part) {
+  double v[2] = {0.0, 0.0};  // sum d^2, max |d|
+  const long long n = rows * nx;
+  const double *fx = tab, *fy = tab + nx, *fz = tab + nx + ny;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / nx, x = i - row * nx;
+    const long long z = row / ny, y = row - z * ny;
+    const double ref = 0.125 * fx[x] * fy[y] * fz[z];
+    const double d = load_elem(f, dt, (size_t)(row * p + x)) - ref;
+    v[0] += d * d;
+    v[1] = fmax(v[1], fabs(d));
+  }
+  block_reduce<2>(v, part, true);
+}
+
+static int ensure_partials(d3h_context *ctx) {
+  if (ctx->partials) return D3H_OK;
+  ctx->partial_blocks = std::max(1, ctx->num_cus) * 4;
+  D3H_HIP(hipMalloc(&ctx->partials, sizeof(double) * 4 * ctx->partial_blocks));
+  D3H_HIP(hipHostMalloc(&ctx->host_partials, sizeof(double) * 4 * ctx->partial_blocks));
+  return D3H_OK;
+}
+
+// ---- HBM probe ----------------------------------------------------------------
+__global__ void __launch_bounds__(256) scale_copy_kernel(const u32x4 *__restrict__ in,
+                                                         u32x4 *__restrict__ out, long long nvec,
+                                                         float cc) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < nvec; i += 4 * stride) {
+    u32x4 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = in[i + j * stride];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      u32x4 o;
+      o.x = __float_as_uint(cc * __uint_as_float(v[j].x));
+      o.y = __float_as_uint(cc * __uint_as_float(v[j].y));
+      o.z = __float_as_uint(cc * __uint_as_float(v[j].z));
+      o.w = __float_as_uint(cc * __uint_as_float(v[j].w));
+      __builtin_nontemporal_store(o, out + i + j * stride);
+    }
+  }
+  for (; i < nvec; i += stride) {
+    const u32x4 v = in[i];
+    u32x4 o;
+    o.x = __float_as_uint(cc * __uint_as_float(v.x));
+    o.y = __float_as_uint(cc * __uint_as_float(v.y));
+    o.z = __float_as_uint(cc * __uint_as_float(v.z));
+    o.w = __float_as_uint(cc * __uint_as_float(v.w));
+    __builtin_nontemporal_store(o, out + i);
+  }
+}
+
+}  // namespace d3h
+
+using namespace d3h;
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+extern "C" {
+
+const char *d3h_status_string(int status) {
+  switch (status) {
+    case D3H_OK: return "ok";
+    case D3H_ERR_INVALID: return "invalid argument";
+    case D3H_ERR_HIP: return "HIP runtime error";
+    case D3H_ERR_NOMEM: return "out of memory";
+    case D3H_ERR_UNSUPPORTED: return "unsupported configuration";
+    case D3H_ERR_STATE: return "invalid state";
+    case D3H_ERR_NOT_FOUND: return "not found";
+    case D3H_ERR_UNSTABLE: return "unstable coefficients";
+    default: return "unknown status";
+  }
+}
+
+const char *d3h_last_error(void) { return t_err.c_str(); }
+
+void d3h_run_config_default(d3h_run_config *cfg) {
+  if (!cfg) return;
+  cfg->temporal_depth = 1;
+  cfg->arith = D3H_ARITH_EXACT;
+  cfg->tile = -1;
+  cfg->zchunks = 0;
+  cfg->use_graph = 0;
+}
+
+int d3h_context_create(int device, d3h_context **out) {
+  if (!out) return fail(D3H_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  D3H_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(D3H_ERR_INVALID, "device %d of %d", device, ndev);
+  D3H_HIP(hipSetDevice(device));
+  auto *c = new d3h_context();
+  c->device = device;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(D3H_ERR_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+  }
+  c->num_cus = prop.multiProcessorCount;
+  c->lds_per_block = (int)prop.sharedMemPerBlock;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    d3h_context_destroy(c);
+    return fail(D3H_ERR_HIP, "stream/event creation failed");
+  }
+  *out = c;
+  return D3H_OK;
+}
+
+int d3h_context_destroy(d3h_context *c) {
+  if (!c) return D3H_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->copy) hipStreamSynchronize(c->copy);
+  if (c->partials) hipFree(c->partials);
+  if (c->host_partials) hipHostFree(c->host_partials);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->copy) hipStreamDestroy(c->copy);
+  delete c;
+  return D3H_OK;
+}
+
+int d3h_device_info(d3h_context *ctx, int *num_cus, size_t *free_bytes, size_t *total_bytes,
+                    int *lds_bytes) {
+  if (!ctx) return fail(D3H_ERR_INVALID, "ctx is NULL");
+  D3H_HIP(hipSetDevice(ctx->device));
+  size_t fr = 0, tot = 0;
+  D3H_HIP(hipMemGetInfo(&fr, &tot));
+  if (num_cus) *num_cus = ctx->num_cus;
+  if (free_bytes) *free_bytes = fr;
+  if (total_bytes) *total_bytes = tot;
+  if (lds_bytes) *lds_bytes = ctx->lds_per_block;
+  return D3H_OK;
+}
+
+int d3h_host_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return fail(D3H_ERR_INVALID, "ptr is NULL");
+  *ptr = nullptr;
+  if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess)
+    return fail(D3H_ERR_NOMEM, "hipHostMalloc(%zu) failed", bytes);
+  return D3H_OK;
+}
+
+int d3h_host_free(void *ptr) {
+  if (ptr) D3H_HIP(hipHostFree(ptr));
+  return D3H_OK;
+}
+
+static void release_graph(d3h_field *f) {
+  if (f->gexec) hipGraphExecDestroy(f->gexec);
+  f->gexec = nullptr;
+  f->gkey.clear();
+}
+
+int d3h_field_create(d3h_context *ctx, int nx, int ny, int nz, int dtype, d3h_field **out) {
+  if (!ctx || !out) return fail(D3H_ERR_INVALID, "ctx/out is NULL");
+  *out = nullptr;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(D3H_ERR_INVALID, "dims %dx%dx%d", nx, ny, nz);
+  if (!dtype_valid(dtype)) return fail(D3H_ERR_INVALID, "dtype %d", dtype);
+  D3H_HIP(hipSetDevice(ctx->device));
+  auto *f = new d3h_field();
+  f->ctx = ctx;
+  f->nx = nx; f->ny = ny; f->nz = nz; f->dtype = dtype;
+  f->pitch = pitch_elems(nx, dtype);
+  f->plane = (size_t)f->pitch * ny;
+  f->elems = f->plane * nz;
+  f->bytes = f->elems * dtype_bytes(dtype);
+  for (int i = 0; i < 2; ++i) {
+    if (hipMalloc(&f->buf[i], f->bytes) != hipSuccess) {
+      d3h_field_destroy(f);
+      return fail(D3H_ERR_NOMEM, "hipMalloc(%zu) for a %dx%dx%d field failed", f->bytes, nx, ny, nz);
+    }
+    if (hipMemsetAsync(f->buf[i], 0, f->bytes, ctx->stream) != hipSuccess) {
+      d3h_field_destroy(f);
+      return fail(D3H_ERR_HIP, "hipMemsetAsync failed");
+    }
+  }
+  D3H_HIP(hipStreamSynchronize(ctx->stream));
+  *out = f;
+  return D3H_OK;
+}
+
+int d3h_field_destroy(d3h_field *f) {
+  if (!f) return D3H_OK;
+  hipSetDevice(f->ctx->device);
+  hipStreamSynchronize(f->ctx->stream);
+  release_graph(f);
+  for (int i = 0; i < 2; ++i)
+    if (f->buf[i]) hipFree(f->buf[i]);
+  if (f->staging) hipFree(f->staging);
+  delete f;
+  return D3H_OK;
+}
+
+int d3h_field_dims(const d3h_field *f, int *nx, int *ny, int *nz, int *dtype) {
+  if (!f) return fail(D3H_ERR_INVALID, "field is NULL");
+  if (nx) *nx = f->nx;
+  if (ny) *ny = f->ny;
+  if (nz) *nz = f->nz;
+  if (dtype) *dtype = f->dtype;
+  return D3H_OK;
+}
+
+static int ensure_staging(d3h_field *f, size_t bytes) {
+  if (f->staging_bytes >= bytes) return D3H_OK;
+  if (f->staging) hipFree(f->staging);
+  f->staging = nullptr;
+  f->staging_bytes = 0;
+  if (hipMalloc(&f->staging, bytes) != hipSuccess)
+    return fail(D3H_ERR_NOMEM, "staging hipMalloc(%zu) failed", bytes);
+  f->staging_bytes = bytes;
+  return D3H_OK;
+}
+
+int d3h_field_upload(d3h_field *f, const void *host, int host_dtype) {
+  if (!f || !host) return fail(D3H_ERR_INVALID, "field/host is NULL");
+  if (!dtype_valid(host_dtype)) return fail(D3H_ERR_INVALID, "host dtype %d", host_dtype);
+  d3h_context *ctx = f->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  const size_t n = (size_t)f->nx * f->ny * f->nz;
+  if (host_dtype == f->dtype) {
+    const size_t es = dtype_bytes(f->dtype);
+    D3H_HIP(hipMemcpy2DAsync(f->buf[f->cur], f->pitch * es, host, f->nx * es, f->nx * es,
+                             (size_t)f->ny * f->nz, hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    D3H_TRY(ensure_staging(f, n * dtype_bytes(host_dtype)));
+    D3H_HIP(hipMemcpyAsync(f->staging, host, n * dtype_bytes(host_dtype), hipMemcpyHostToDevice,
+                           ctx->stream));
+    D3H_TRY(launch_convert(f->staging, host_dtype, 0, f->buf[f->cur], f->dtype, f->pitch, f->nx, f->ny,
+                           f->nz, ctx->stream));
+  }
+  D3H_HIP(hipStreamSynchronize(ctx->stream));
+  return D3H_OK;
+}
+
+int d3h_field_download(d3h_field *f, void *host, int host_dtype) {
+  if (!f || !host) return fail(D3H_ERR_INVALID, "field/host is NULL");
+  if (!dtype_valid(host_dtype)) return fail(D3H_ERR_INVALID, "host dtype %d", host_dtype);
+  d3h_context *ctx = f->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  const size_t n = (size_t)f->nx * f->ny * f->nz;
+  if (host_dtype == f->dtype) {
+    const size_t es = dtype_bytes(f->dtype);
+    D3H_HIP(hipMemcpy2DAsync(host, f->nx * es, f->buf[f->cur], f->pitch * es, f->nx * es,
+                             (size_t)f->ny * f->nz, hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    D3H_TRY(ensure_staging(f, n * dtype_bytes(host_dtype)));
+    D3H_TRY(launch_convert(f->buf[f->cur], f->dtype, f->pitch, f->staging, host_dtype, 0, f->nx, f->ny,
+                           f->nz, ctx->stream));
+    D3H_HIP(hipMemcpyAsync(host, f->staging, n * dtype_bytes(host_dtype), hipMemcpyDeviceToHost,
+                           ctx->stream));
+  }
+  D3H_HIP(hipStreamSynchronize(ctx->stream));
+  return D3H_OK;
+}
+
+int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_config *cfg_in,
+                  d3h_timing *timing) {
+  const double t0 = now_ms();
+  if (!f || !coef) return fail(D3H_ERR_INVALID, "field/coef is NULL");
+  if (count < 0) return fail(D3H_ERR_INVALID, "count %d < 0", count);
+  d3h_run_config cfg;
+  d3h_run_config_default(&cfg);
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.temporal_depth <= 0) cfg.temporal_depth = 1;
+  d3h_context *ctx = f->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  const KernelEntry *e = nullptr;
+  int zchunks = 0;
+  D3H_TRY(resolve_entry(ctx, f->nx, f->ny, f->nz, f->dtype, cfg, &e, &zchunks));
+  int launches = 0;
+  unsigned blocks = 0;
+  float ms = 0.f;
+  if (count > 0) {
+    if (cfg.use_graph) {
+      std::vector<long long> key = {count, (long long)(intptr_t)e, zchunks, f->cur};
+      for (int i = 0; i < 7; ++i) {
+        long long bits;
+        memcpy(&bits, &coef[i], 8);
+        key.push_back(bits);
+      }
+      if (key != f->gkey) {
+        release_graph(f);
+        const int cur0 = f->cur;
+        hipGraph_t graph = nullptr;
+        D3H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_steps(f, count, coef, *e, zchunks, ctx->stream, &launches, &blocks);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        f->cur = cur0;
+        if (rc != D3H_OK) {
+          if (graph) hipGraphDestroy(graph);
+          return rc;
+        }
+        if (ce != hipSuccess) return fail(D3H_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&f->gexec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(D3H_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        key.push_back(launches);
+        key.push_back(blocks);
+        f->gkey = key;
+      }
+      launches = (int)f->gkey[f->gkey.size() - 2];
+      blocks = (unsigned)f->gkey.back();
+      D3H_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+      D3H_HIP(hipGraphLaunch(f->gexec, ctx->stream));
+      D3H_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+      if (launches & 1) f->cur ^= 1;
+    } else {
+      D3H_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+      D3H_TRY(enqueue_steps(f, count, coef, *e, zchunks, ctx->stream, &launches, &blocks));
+      D3H_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    }
+    D3H_HIP(hipEventSynchronize(ctx->ev1));
+    D3H_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  }
+  if (timing) {
+    timing->kernel_ms = ms;
+    timing->launches = launches;
+    timing->avg_launch_ms = launches ? ms / launches : 0.0;
+    timing->steps = count;
+    timing->temporal_depth = e->T;
+    timing->tile = (int)(e - registry().data());
+    Geometry g{};
+    unsigned nb = 0;
+    if (make_geometry(*e, f->nx, f->ny, f->nz, f->pitch, zchunks, ctx->num_cus, occupancy_of(*e), &g,
+                      &nb) == D3H_OK) {
+      timing->zchunks = g.nzc;
+      timing->blocks = (int)nb;
+    }
+    timing->wall_ms = now_ms() - t0;
+  }
+  return D3H_OK;
+}
+
+int d3h_field_compare(const d3h_field *a, const d3h_field *b, double out[3]) {
+  if (!a || !b || !out) return fail(D3H_ERR_INVALID, "NULL argument");
+  if (a->nx != b->nx || a->ny != b->ny || a->nz != b->nz)
+    return fail(D3H_ERR_INVALID, "dims differ");
+  d3h_context *ctx = a->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  D3H_TRY(ensure_partials(ctx));
+  const int nb = ctx->partial_blocks;
+  hipLaunchKernelGGL(compare_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream, a->buf[a->cur],
+                     a->dtype, a->pitch, b->buf[b->cur], b->dtype, b->pitch, a->nx,
+                     (long long)a->ny * a->nz, ctx->partials);
+  D3H_HIP(hipGetLastError());
+  D3H_HIP(hipMemcpyAsync(ctx->host_partials, ctx->partials, sizeof(double) * 3 * nb,
+                         hipMemcpyDeviceToHost, ctx->stream));
+  D3H_HIP(hipStreamSynchronize(ctx->stream));
+  double d2 = 0, b2 = 0, mx = 0;
+  for (int i = 0; i < nb; ++i) {
+    d2 += ctx->host_partials[3 * i];
+    b2 += ctx->host_partials[3 * i + 1];
+    mx = std::max(mx, ctx->host_partials[3 * i + 2]);
+  }
+  out[0] = b2 > 0 ? std::sqrt(d2 / b2) : std::sqrt(d2);
+  out[1] = mx;
+  out[2] = std::sqrt(d2);
+  return D3H_OK;
+}
+
+int d3h_field_error_analytical(const d3h_field *f, double kappa, double time, const double k[3],
+                               const double d[3], double out[2]) {
+  if (!f || !k || !d || !out) return fail(D3H_ERR_INVALID, "NULL argument");
+  d3h_context *ctx = f->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  D3H_TRY(ensure_partials(ctx));
+  const int dims[3] = {f->nx, f->ny, f->nz};
+  std::vector<double> tab((size_t)f->nx + f->ny + f->nz);
+  size_t o = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double amp = std::exp(-kappa * time * k[a] * k[a]);
+    for (int j = 0; j < dims[a]; ++j) tab[o++] = 1.0 - amp * std::cos(k[a] * (d[a] * (j + 0.5)));
+  }
+  double *dtab = nullptr;
+  D3H_HIP(hipMalloc(&dtab, sizeof(double) * tab.size()));
+  hipError_t e = hipMemcpy(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(dtab);
+    return fail(D3H_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+  }
+  const int nb = ctx->partial_blocks;
+  hipLaunchKernelGGL(analytical_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream, f->buf[f->cur],
+                     f->dtype, f->pitch, f->nx, f->ny, (long long)f->ny * f->nz, dtab, ctx->partials);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->host_partials, ctx->partials, sizeof(double) * 2 * nb,
+                       hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(dtab);
+  if (e != hipSuccess) return fail(D3H_ERR_HIP, "analytical norm: %s", hipGetErrorString(e));
+  double d2 = 0, mx = 0;
+  for (int i = 0; i < nb; ++i) {
+    d2 += ctx->host_partials[2 * i];
+    mx = std::max(mx, ctx->host_partials[2 * i + 1]);
+  }
+  out[0] = std::sqrt(d2 / ((double)f->nx * f->ny * f->nz));
+  out[1] = mx;
+  return D3H_OK;
+}
+
+int d3h_tile_count(void) { return (int)registry().size(); }
+
+int d3h_tile_info(int index, d3h_tile_desc *out) {
+  if (!out) return fail(D3H_ERR_INVALID, "out is NULL");
+  const auto &reg = registry();
+  if (index < 0 || index >= (int)reg.size()) return fail(D3H_ERR_NOT_FOUND, "tile %d", index);
+  const KernelEntry &e = reg[index];
+  out->dtype = e.dtype;
+  out->temporal_depth = e.T;
+  out->arith = e.fma ? D3H_ARITH_FMA : D3H_ARITH_EXACT;
+  out->groups = e.G;
+  out->rows_per_wave = e.RPW;
+  out->waves = e.NW;
+  out->tile_x = e.tile_x();
+  out->tile_y = e.tile_y();
+  return D3H_OK;
+}
+
+int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int temporal_depth, int arith,
+                 int *best_tile, int *best_zchunks, double *best_ms_per_step) {
+  if (!ctx) return fail(D3H_ERR_INVALID, "ctx is NULL");
+  const int T = temporal_depth <= 0 ? 1 : temporal_depth;
+  const int fma = arith == D3H_ARITH_FMA ? 1 : 0;
+  d3h_field *f = nullptr;
+  D3H_TRY(d3h_field_create(ctx, nx, ny, nz, dtype, &f));
+  const double coef[7] = {0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4};
+  const auto &reg = registry();
+  double best = 1e300;
+  int bi = -1, bz = 0;
+  const long long cells = (long long)nx * ny * nz;
+  // enough launches for ~2 ms of work at ~1 Tcell/s, at least 4
+  const int reps = (int)std::max(4LL, std::min(200LL, 2000000000LL / std::max(cells, 1LL) / T));
+  int rc = D3H_OK;
+  for (int i = 0; i < (int)reg.size() && rc == D3H_OK; ++i) {
+    const KernelEntry &e = reg[i];
+    if (e.dtype != dtype || e.T != T || e.fma != fma) continue;
+    const int occ = occupancy_of(e);
+    if (occ <= 0) continue;
+    Geometry g{};
+    unsigned nb = 0;
+    if (make_geometry(e, nx, ny, nz, f->pitch, 0, ctx->num_cus, occ, &g, &nb) != D3H_OK) continue;
+    std::vector<int> cands = {g.nzc, g.nzc * 2, g.nzc * 4, std::max(1, g.nzc / 2)};
+    std::sort(cands.begin(), cands.end());
+    cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+    for (int zc : cands) {
+      if (zc > nz) continue;
+      Geometry g2{};
+      unsigned nb2 = 0;
+      if (make_geometry(e, nx, ny, nz, f->pitch, zc, ctx->num_cus, occ, &g2, &nb2) != D3H_OK) continue;
+      e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);  // warm-up
+      f->cur ^= 1;
+      hipEventRecord(ctx->ev0, ctx->stream);
+      for (int r = 0; r < reps; ++r) {
+        e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);
+        f->cur ^= 1;
+      }
+      hipEventRecord(ctx->ev1, ctx->stream);
+      if (hipEventSynchronize(ctx->ev1) != hipSuccess || hipGetLastError() != hipSuccess) {
+        rc = fail(D3H_ERR_HIP, "autotune launch of tile %d failed", i);
+        break;
+      }
+      float ms = 0;
+      hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+      const double per_step = ms / reps / T;
+      if (per_step < best) {
+        best = per_step;
+        bi = i;
+        bz = g2.nzc;
+      }
+    }
+  }
+  d3h_field_destroy(f);
+  if (rc != D3H_OK) return rc;
+  if (bi < 0) return fail(D3H_ERR_UNSUPPORTED, "no kernel for dtype %d depth %d", dtype, T);
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const KernelEntry &e = reg[bi];
+    ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] = TuneVal{e.vec, e.G, e.RPW, e.NW, bz, best};
+  }
+  if (best_tile) *best_tile = bi;
+  if (best_zchunks) *best_zchunks = bz;
+  if (best_ms_per_step) *best_ms_per_step = best;
+  return D3H_OK;
+}
+
+int d3h_autotune_save(d3h_context *ctx, const char *path) {
+  if (!ctx || !path) return fail(D3H_ERR_INVALID, "NULL argument");
+  std::ofstream os(path);
+  if (!os) return fail(D3H_ERR_INVALID, "cannot write %s", path);
+  os << "# dtype depth fma log2nx log2ny log2nz vec groups rows_per_wave waves zchunks ms_per_step\n";
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  for (const auto &kv : ctx->tuned) {
+    const auto &k = kv.first;
+    const auto &v = kv.second;
+    os << k.dtype << ' ' << k.T << ' ' << k.fma << ' ' << k.cx << ' ' << k.cy << ' ' << k.cz << ' '
+       << v.vec << ' ' << v.G << ' ' << v.RPW << ' ' << v.NW << ' ' << v.zchunks << ' '
+       << v.ms_per_step << '\n';
+  }
+  return os ? D3H_OK : fail(D3H_ERR_INVALID, "write to %s failed", path);
+}
+
+int d3h_autotune_load(d3h_context *ctx, const char *path) {
+  if (!ctx || !path) return fail(D3H_ERR_INVALID, "NULL argument");
+  std::ifstream is(path);
+  if (!is) return fail(D3H_ERR_NOT_FOUND, "cannot read %s", path);
+  std::string line;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  while (std::getline(is, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream ss(line);
+    TuneKey k{};
+    TuneVal v{};
+    if (!(ss >> k.dtype >> k.T >> k.fma >> k.cx >> k.cy >> k.cz >> v.vec >> v.G >> v.RPW >> v.NW >>
+          v.zchunks >> v.ms_per_step))
+      return fail(D3H_ERR_INVALID, "malformed line in %s: %s", path, line.c_str());
+    if (find_entry(k.dtype, k.T, k.fma, v.vec, v.G, v.RPW, v.NW)) ctx->tuned[k] = v;
+  }
+  return D3H_OK;
+}
+
+int d3h_bandwidth_probe(d3h_context *ctx, size_t n, int dtype, int reps, double *gbps, double *avg_ms) {
+  if (!ctx) return fail(D3H_ERR_INVALID, "ctx is NULL");
+  if (!dtype_valid(dtype) || reps <= 0 || n == 0) return fail(D3H_ERR_INVALID, "bad probe arguments");
+  D3H_HIP(hipSetDevice(ctx->device));
+  const size_t bytes = (n * dtype_bytes(dtype) + 15) / 16 * 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return fail(D3H_ERR_NOMEM, "probe alloc");
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    return fail(D3H_ERR_NOMEM, "probe alloc");
+  }
+  hipMemsetAsync(a, 0, bytes, ctx->stream);
+  const long long nvec = (long long)(bytes / 16);
+  const unsigned blocks = (unsigned)std::max(1, ctx->num_cus) * 8;
+  hipLaunchKernelGGL(scale_copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (const u32x4 *)a,
+                     (u32x4 *)b, nvec, 0.4f);
+  hipEventRecord(ctx->ev0, ctx->stream);
+  for (int r = 0; r < reps; ++r) {
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                       (const u32x4 *)((r & 1) ? b : a), (u32x4 *)((r & 1) ? a : b), nvec, 0.4f);
+  }
+  hipEventRecord(ctx->ev1, ctx->stream);
+  hipError_t e = hipEventSynchronize(ctx->ev1);
+  if (e == hipSuccess) e = hipGetLastError();
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return fail(D3H_ERR_HIP, "probe: %s", hipGetErrorString(e));
+  const double avg = ms / reps;
+  if (avg_ms) *avg_ms = avg;
+  if (gbps) *gbps = 2.0 * (double)bytes / (avg * 1e-3) * 1e-9;
+  return D3H_OK;
+}
+
+// ---- Physis-shaped drop-in ----------------------------------------------------
+static struct DropIn {
+  d3h_context *ctx = nullptr;
+  d3h_field *field = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+  float kernel_seconds = 0.f;
+} g_drop;
+
+static int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return (v && *v) ? atoi(v) : dflt;
+}
+
+int initialize_hip(int argc, char **argv, int nx, int ny, int nz) {
+  (void)argc;
+  (void)argv;
+  if (g_drop.ctx) return fail(D3H_ERR_STATE, "initialize_hip called twice");
+  if (nx <= 0 || ny <= 0 || nz <= 0) return fail(D3H_ERR_INVALID, "dims %dx%dx%d", nx, ny, nz);
+  D3H_TRY(d3h_context_create(env_int("D3H_DEVICE", 0), &g_drop.ctx));
+  g_drop.nx = nx; g_drop.ny = ny; g_drop.nz = nz;
+  return D3H_OK;
+}
+
+int initialize_benchmark_hip(int nx, int ny, int nz) {
+  if (!g_drop.ctx) return fail(D3H_ERR_STATE, "initialize_hip not called");
+  if (nx != g_drop.nx || ny != g_drop.ny || nz != g_drop.nz)
+    return fail(D3H_ERR_INVALID, "dims differ from initialize_hip");
+  if (g_drop.field) return fail(D3H_ERR_STATE, "benchmark already initialised");
+  return d3h_field_create(g_drop.ctx, nx, ny, nz, D3H_F32, &g_drop.field);
+}
+
+int run_kernel_hip(int count, float *f1_host, int nx, int ny, int nz, float ce, float cw, float cn,
+                   float cs, float ct, float cb, float cc) {
+  if (!g_drop.field) return fail(D3H_ERR_STATE, "initialize_benchmark_hip not called");
+  if (!f1_host) return fail(D3H_ERR_INVALID, "f1_host is NULL");
+  if (nx != g_drop.nx || ny != g_drop.ny || nz != g_drop.nz)
+    return fail(D3H_ERR_INVALID, "dims differ from initialize_hip");
+  const double coef[7] = {ce, cw, cn, cs, ct, cb, cc};
+  d3h_run_config cfg;
+  d3h_run_config_default(&cfg);
+  cfg.temporal_depth = env_int("D3H_TEMPORAL_DEPTH", 1);
+  cfg.arith = env_int("D3H_ARITH", D3H_ARITH_EXACT);
+  cfg.use_graph = env_int("D3H_USE_GRAPH", 0);
+  D3H_TRY(d3h_field_upload(g_drop.field, f1_host, D3H_F32));
+  d3h_timing t{};
+  D3H_TRY(d3h_field_run(g_drop.field, count, coef, &cfg, &t));
+  g_drop.kernel_seconds = (float)(t.kernel_ms * 1e-3);
+  return d3h_field_download(g_drop.field, f1_host, D3H_F32);
+}
+
+int finalize_benchmark_hip(void) {
+  d3h_field_destroy(g_drop.field);
+  g_drop.field = nullptr;
+  d3h_context_destroy(g_drop.ctx);
+  g_drop.ctx = nullptr;
+  return D3H_OK;
+}
+
+int kernel_time_hip(float *seconds) {
+  if (!seconds) return fail(D3H_ERR_INVALID, "seconds is NULL");
+  *seconds = g_drop.kernel_seconds;
+  return D3H_OK;
+}
+
+}  // extern "C"

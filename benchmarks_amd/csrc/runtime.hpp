@@ -1,0 +1,104 @@
+// Internal runtime types shared by runtime.hip, batch.hip and server.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "diffusion3d_hip.h"
+#include "kernels.hpp"
+
+namespace d3h {
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define D3H_HIP(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::d3h::fail(D3H_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                              \
+  } while (0)
+
+#define D3H_TRY(expr)             \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != D3H_OK) return rc_; \
+  } while (0)
+
+size_t dtype_bytes(int dtype);
+bool dtype_valid(int dtype);
+// Row pitch in elements: nx rounded up to a multiple of 16 bytes.
+int pitch_elems(int nx, int dtype);
+
+// All compiled kernels.
+const std::vector<KernelEntry> &registry();
+// Default entry for (dtype, T, fma) (first in table order), or nullptr.
+const KernelEntry *default_entry(int dtype, int T, int fma);
+
+// Geometry and block count of one launch of `e` over a grid; zchunks 0 =
+// heuristic from occupancy and CU count.
+int make_geometry(const KernelEntry &e, int nx, int ny, int nz, int pitch, int zchunks,
+                  int num_cus, int occupancy, Geometry *g, unsigned *blocks);
+int occupancy_of(const KernelEntry &e);
+
+struct TuneKey {
+  int dtype, T, fma, cx, cy, cz;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(dtype, T, fma, cx, cy, cz) < std::tie(o.dtype, o.T, o.fma, o.cx, o.cy, o.cz);
+  }
+};
+struct TuneVal {
+  int vec, G, RPW, NW;  // entry identity (stable across builds)
+  int zchunks;
+  double ms_per_step;
+};
+// Size class of a grid: ceil(log2) of each dimension.
+TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz);
+
+}  // namespace d3h
+
+struct d3h_context {
+  int device = 0;
+  hipStream_t stream = nullptr;  // compute
+  hipStream_t copy = nullptr;    // host transfers (serving)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cus = 0;
+  int lds_per_block = 0;
+  double *partials = nullptr;       // device scratch for reductions
+  double *host_partials = nullptr;  // pinned mirror
+  int partial_blocks = 0;
+  std::mutex mu;                    // guards `tuned`
+  std::map<d3h::TuneKey, d3h::TuneVal> tuned;
+};
+
+struct d3h_field {
+  d3h_context *ctx = nullptr;
+  int nx = 0, ny = 0, nz = 0, dtype = 0, pitch = 0;
+  size_t plane = 0, elems = 0, bytes = 0;
+  void *buf[2] = {nullptr, nullptr};
+  int cur = 0;
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  // cached captured launch sequence
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> gkey;
+};
+
+namespace d3h {
+// Resolve the kernel for a run configuration (tuned / explicit / default).
+int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
+                  const KernelEntry **entry, int *zchunks);
+// Enqueue `count` steps from f->buf[f->cur] on `stream` (no sync); updates cur.
+int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEntry &e, int zchunks,
+                  hipStream_t stream, int *launches, unsigned *blocks);
+// Convert nx*ny*nz cells between device arrays of two dtypes; a row pitch of
+// 0 means dense (pitch = nx).  Narrowing rounds to nearest even (from f64 to
+// bf16/f16 it rounds to f32 first).
+int launch_convert(const void *src, int src_dtype, int src_pitch, void *dst, int dst_dtype,
+                   int dst_pitch, int nx, int ny, int nz, hipStream_t stream);
+}  // namespace d3h

@@ -1,0 +1,476 @@
+// 3D 7-point diffusion step kernels for gfx950 (CDNA4, wave64).
+//
+// One template covers the plain step (T = 1) and temporal blocking (T > 1
+// fused steps per launch) for FP32, FP64 and BF16/FP16 storage.
+//
+// Work decomposition
+//   * A workgroup owns an xy tile and a z-chunk.  The tile is TILE_X = 64 lanes
+//     x G groups x VEC cells wide (VEC cells = one 16- or 8-byte vector) and
+//     TILE_Y = NW waves x RPW rows tall.  Lane l of group g holds cells
+//     x0 + g*64*VEC + l*VEC + [0, VEC) of each of its wave's RPW consecutive
+//     rows, so every HBM access is a coalesced 128-bit (or 64-bit) access.
+//   * The workgroup streams z: each iteration loads one plane (one iteration
+//     ahead of use) into registers.  Stage s (1..T) then computes time level s
+//     at plane i - s from level s-1 planes (i-s-1, i-s, i-s+1), all held in
+//     registers; plane q of a level lives in register slot q mod 4, and the
+//     z-loop is unrolled by 4 so slots are compile-time and never moved.
+//   * x neighbours: inside a lane from registers, across lanes by DPP
+//     wave_shr:1 / wave_shl:1 (group seams via readlane).  y neighbours: from
+//     the wave's own rows in registers; a wave's first/last rows are exchanged
+//     with the adjacent waves through LDS (double-buffered, one barrier per
+//     plane).  Tile edges use the cell itself, which is exactly the clamped
+//     boundary at a domain edge; at interior tile edges the resulting invalid
+//     cells are never stored (tiles overlap by the halo, see Geometry).
+//   * Clamp selects (x = nx-1, y = ny-1, z = 0 / nz-1) are only evaluated in
+//     the EDGE variant of a stage, used for tiles that contain the grid's last
+//     column or row beyond their own last cell, and for the first/last plane.
+//   * Arithmetic: EXACT evaluates cc*c + cw*w + ce*e + cs*s + cn*n + cb*b +
+//     ct*t left to right with separately rounded operations (the reference's
+//     order, diffusion3d/baseline.cc:39-40; the file is built with
+//     -ffp-contract=off), so results are bitwise equal to the reference CPU
+//     path.  FMA uses fma(cc,c,cw*w) followed by an fma chain.
+//   * Reduced-precision storage rounds every time level (also the fused
+//     intermediate ones) to storage precision, so T fused steps are bitwise
+//     equal to T single steps for every dtype.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace d3h {
+
+enum StoreKind : int { kF32 = 0, kF64 = 1, kBF16 = 2, kF16 = 3 };
+
+template <int S> struct Traits;
+template <> struct Traits<kF32> { using T = float; using C = float; };
+template <> struct Traits<kF64> { using T = double; using C = double; };
+template <> struct Traits<kBF16> { using T = uint16_t; using C = float; };
+template <> struct Traits<kF16> { using T = uint16_t; using C = float; };
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Launch geometry, computed on the host (make_geometry in runtime.cpp).
+//   Tiles overlap: tile (tx,ty,tz) stores x in [tx*ox, tx*ox+ox), rows
+//   [ty*oy, ty*oy+oy), planes [tz*oz, tz*oz+oz) (clipped to the grid) and loads
+//   a window that extends hx cells / T rows / T planes beyond that on every
+//   side that is not a domain boundary.  pitch is the padded row length
+//   (multiple of 16 bytes); plane = pitch * ny.
+struct Geometry {
+  int nx, ny, nz, pitch;
+  long long plane;
+  int ntx, nty, nzc;
+  int ox, oy, oz, hx;
+};
+
+template <typename C> struct Coef { C ce, cw, cn, cs, ct, cb, cc; };
+
+// ---- storage conversion (round-to-nearest-even) ----------------------------
+__device__ __forceinline__ uint32_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t h) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(h & 0xffffu));
+}
+__device__ __forceinline__ uint32_t to_f16(float f) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+// VEC storage elements <-> NWORDS 32-bit words.
+template <int S, int VEC>
+struct Pack {
+  using C = typename Traits<S>::C;
+  static constexpr int kBytes = VEC * (int)sizeof(typename Traits<S>::T);
+  static constexpr int kWords = kBytes / 4;
+  static_assert(kBytes == 8 || kBytes == 16, "lane vectors are 8 or 16 bytes");
+
+  __device__ static __forceinline__ void unpack(const uint32_t *w, C *v) {
+    if constexpr (S == kF32) {
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) v[i] = __uint_as_float(w[i]);
+    } else if constexpr (S == kF64) {
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) v[i] = __hiloint2double((int)w[2 * i + 1], (int)w[2 * i]);
+    } else if constexpr (S == kBF16) {
+#pragma unroll
+      for (int i = 0; i < kWords; ++i) {
+        v[2 * i] = __uint_as_float(w[i] << 16);
+        v[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kWords; ++i) {
+        v[2 * i] = f16_bits_to_f32(w[i]);
+        v[2 * i + 1] = f16_bits_to_f32(w[i] >> 16);
+      }
+    }
+  }
+  __device__ static __forceinline__ void pack(const C *v, uint32_t *w) {
+    if constexpr (S == kF32) {
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) w[i] = __float_as_uint(v[i]);
+    } else if constexpr (S == kF64) {
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) {
+        w[2 * i] = (uint32_t)__double2loint(v[i]);
+        w[2 * i + 1] = (uint32_t)__double2hiint(v[i]);
+      }
+    } else if constexpr (S == kBF16) {
+#pragma unroll
+      for (int i = 0; i < kWords; ++i) w[i] = to_bf16(v[2 * i]) | (to_bf16(v[2 * i + 1]) << 16);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kWords; ++i) w[i] = to_f16(v[2 * i]) | (to_f16(v[2 * i + 1]) << 16);
+    }
+  }
+  __device__ static __forceinline__ void load(const void *p, C *v) {
+    uint32_t w[kWords];
+    if constexpr (kBytes == 16) {
+      const u32x4 u = *reinterpret_cast<const u32x4 *>(p);
+      w[0] = u.x; w[1] = u.y; w[2] = u.z; w[3] = u.w;
+    } else {
+      const u32x2 u = *reinterpret_cast<const u32x2 *>(p);
+      w[0] = u.x; w[1] = u.y;
+    }
+    unpack(w, v);
+  }
+  __device__ static __forceinline__ void store(void *p, const C *v) {
+    uint32_t w[kWords];
+    pack(v, w);
+    if constexpr (kBytes == 16) {
+      u32x4 u = {w[0], w[1], w[2], w[3]};
+      __builtin_nontemporal_store(u, reinterpret_cast<u32x4 *>(p));
+    } else {
+      u32x2 u = {w[0], w[1]};
+      __builtin_nontemporal_store(u, reinterpret_cast<u32x2 *>(p));
+    }
+  }
+};
+
+// Round a compute value to storage precision and back (identity for f32/f64).
+template <int S>
+__device__ __forceinline__ typename Traits<S>::C quantize(typename Traits<S>::C x) {
+  if constexpr (S == kBF16) return __uint_as_float(to_bf16(x) << 16);
+  else if constexpr (S == kF16) return f16_bits_to_f32(to_f16(x));
+  else return x;
+}
+
+// ---- cross-lane moves -------------------------------------------------------
+// DPP wave_shr:1 (lane l <- lane l-1) and wave_shl:1 (lane l <- lane l+1);
+// the lane without a source keeps `old`.
+constexpr int kWaveShr1 = 0x138;
+constexpr int kWaveShl1 = 0x130;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float old, float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src),
+                                                    CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double old, double src) {
+  int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(src), CTRL, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(src), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float readlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane(double v, int l) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// The 7-point update; argument order follows the reference expression
+// (s = y+1, n = y-1, b = z-1, t = z+1).
+template <typename C, bool FMA>
+__device__ __forceinline__ C update(const Coef<C> &k, C c, C w, C e, C s, C n, C b, C t) {
+  if constexpr (FMA) {
+    C r = fma_(k.cc, c, k.cw * w);
+    r = fma_(k.ce, e, r);
+    r = fma_(k.cs, s, r);
+    r = fma_(k.cn, n, r);
+    r = fma_(k.cb, b, r);
+    return fma_(k.ct, t, r);
+  } else {
+    return k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
+  }
+}
+
+constexpr int mod4(int v) { return ((v % 4) + 4) % 4; }
+
+// Exchange buffer between the waves of a workgroup: [buf][level][wave][side][v][lane].
+// Double-buffered (one barrier per plane) when both buffers fit in 64 KB,
+// otherwise single-buffered with a second barrier before each publish.
+template <typename C, int T, int NV, int NW>
+struct XchLayout {
+  static constexpr int kPerBuf = T * NW * 2 * NV * 64;
+  static constexpr int kBufs = (2 * kPerBuf * (int)sizeof(C) <= 65536) ? 2 : 1;
+  static constexpr int kElems = (NW > 1) ? kBufs * kPerBuf : 1;
+  __device__ static constexpr int idx(int buf, int lvl, int wave, int side, int v, int lane) {
+    return ((((buf * T + lvl) * NW + wave) * 2 + side) * NV + v) * 64 + lane;
+  }
+};
+
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+struct TileRunner {
+  using ST = typename Traits<S>::T;
+  using C = typename Traits<S>::C;
+  static constexpr int NV = G * VEC;
+  using X = XchLayout<C, T, NV, NW>;
+  using P = Pack<S, VEC>;
+  static constexpr int GSTRIDE = 64 * VEC;  // cells between a lane's groups
+  static constexpr int TILE_X = G * GSTRIDE;
+  static constexpr int TILE_Y = NW * RPW;
+
+  // lv[level][slot][row][cell]: level 0 is the loaded input.
+  C lv[T][4][RPW][NV];
+
+  const ST *__restrict__ src;
+  ST *__restrict__ dst;
+  const Geometry &geo;
+  const Coef<C> &k;
+  C *xch;
+  int lane, wave;
+  int xlane;            // x of this lane's cell 0 in group 0
+  int zl0, zl1, zo0, zo1;
+  bool edge_xy;         // tile holds the last column/row strictly inside it
+  long long rowoff[RPW];
+  bool rowin[RPW], rowst[RPW], rowlast[RPW];
+  bool colin[G], colst[G];
+
+  __device__ __forceinline__ TileRunner(const ST *s_, ST *d_, const Geometry &g, const Coef<C> &k_,
+                                        C *x_, int tx, int ty, int tz)
+      : src(s_), dst(d_), geo(g), k(k_), xch(x_) {
+    lane = __lane_id();
+    wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int xo0 = tx * geo.ox;
+    const int xo1 = min(xo0 + geo.ox, geo.nx);
+    const int xo1s = (xo1 == geo.nx) ? geo.pitch : xo1;  // row padding may be overwritten
+    const int xl0 = (geo.ntx == 1) ? 0 : max(xo0 - geo.hx, 0);
+    const int yo0 = ty * geo.oy;
+    const int yo1 = min(yo0 + geo.oy, geo.ny);
+    const int yl0 = (geo.nty == 1) ? 0 : max(yo0 - T, 0);
+    zo0 = tz * geo.oz;
+    zo1 = min(zo0 + geo.oz, geo.nz);
+    zl0 = max(zo0 - T, 0);
+    zl1 = min(zo1 + T, geo.nz);
+    edge_xy = (xl0 + TILE_X > geo.nx) || (yl0 + TILE_Y > geo.ny);
+    xlane = xl0 + lane * VEC;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int xv = xlane + g * GSTRIDE;
+      colin[g] = xv < geo.pitch;
+      colst[g] = xv >= xo0 && xv < xo1s;
+    }
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int y = yl0 + wave * RPW + r;
+      rowin[r] = y < geo.ny;
+      rowst[r] = y >= yo0 && y < yo1;
+      rowlast[r] = y == geo.ny - 1;
+      rowoff[r] = (long long)y * geo.pitch + xlane;
+    }
+#pragma unroll
+    for (int l = 0; l < T; ++l)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+          for (int v = 0; v < NV; ++v) lv[l][s][r][v] = C(0);
+  }
+
+  template <int SL>
+  __device__ __forceinline__ void load_plane(int q) {
+    const ST *base = src + (long long)q * geo.plane;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (rowin[r] && colin[g]) P::load(base + rowoff[r] + g * GSTRIDE, &lv[0][SL][r][g * VEC]);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store_plane(int q, C (&out)[RPW][NV]) {
+    ST *base = dst + (long long)q * geo.plane;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (rowst[r] && colst[g]) P::store(base + rowoff[r] + g * GSTRIDE, &out[r][g * VEC]);
+      }
+    }
+  }
+
+  // First / last valid plane of level s in this chunk.
+  __device__ __forceinline__ int lo(int s) const { return zl0 > 0 ? zl0 + s : 0; }
+  __device__ __forceinline__ int hi(int s) const { return zl1 < geo.nz ? zl1 - 1 - s : geo.nz - 1; }
+
+  // Publish the first/last row of every level's centre plane for iteration
+  // phase PH (plane i - s of level s - 1 for stage s).
+  template <int PH>
+  __device__ __forceinline__ void publish() {
+    if constexpr (NW > 1) {
+      constexpr int buf = (X::kBufs == 2) ? (PH & 1) : 0;
+#pragma unroll
+      for (int s = 1; s <= T; ++s) {
+        const int sc = mod4(PH - s);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          xch[X::idx(buf, s - 1, wave, 0, v, lane)] = lv[s - 1][sc][0][v];
+          xch[X::idx(buf, s - 1, wave, 1, v, lane)] = lv[s - 1][sc][RPW - 1][v];
+        }
+      }
+    }
+  }
+
+  template <int PH, int s, bool EDGE>
+  __device__ __forceinline__ void stage_impl(int p) {
+    constexpr int sc = mod4(PH - s), sb = mod4(PH - s - 1), st = mod4(PH - s + 1);
+    constexpr int buf = (X::kBufs == 2) ? (PH & 1) : 0;
+    const bool zb = (p == 0), zt = (p == geo.nz - 1);
+    const int wn = max(wave - 1, 0), ws = min(wave + 1, NW - 1);
+    C out[RPW][NV];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const C *cv = lv[s - 1][sc][r];
+      // rows above / below this wave's first / last row
+      C nrow[NV], srow[NV];
+      if (r == 0) {
+        if constexpr (NW > 1) {
+          if (wave > 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) nrow[v] = xch[X::idx(buf, s - 1, wn, 1, v, lane)];
+          } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) nrow[v] = cv[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) nrow[v] = cv[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) nrow[v] = lv[s - 1][sc][r - 1][v];
+      }
+      if (r == RPW - 1) {
+        if constexpr (NW > 1) {
+          if (wave < NW - 1) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) srow[v] = xch[X::idx(buf, s - 1, ws, 0, v, lane)];
+          } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) srow[v] = cv[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) srow[v] = cv[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) srow[v] = lv[s - 1][sc][r + 1][v];
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        // x neighbours across lanes (group seams through readlane)
+        const C wold = (g > 0) ? readlane(cv[(g - 1) * VEC + VEC - 1], 63) : cv[0];
+        const C eold = (g < G - 1) ? readlane(cv[(g + 1) * VEC], 0) : cv[g * VEC + VEC - 1];
+        const C wl = dpp<kWaveShr1>(wold, cv[g * VEC + VEC - 1]);
+        const C er = dpp<kWaveShl1>(eold, cv[g * VEC]);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const int gv = g * VEC + v;
+          const C c = cv[gv];
+          const C w = (v > 0) ? cv[gv - 1] : wl;
+          C e = (v < VEC - 1) ? cv[gv + 1] : er;
+          C so = srow[gv];
+          C b = lv[s - 1][sb][r][gv];
+          C t = lv[s - 1][st][r][gv];
+          if constexpr (EDGE) {
+            if (xlane + g * GSTRIDE + v == geo.nx - 1) e = c;
+            if (rowlast[r]) so = c;
+            if (zb) b = c;
+            if (zt) t = c;
+          }
+          out[r][gv] = update<C, FMA>(k, c, w, e, so, nrow[gv], b, t);
+        }
+      }
+    }
+    if constexpr (s < T) {
+#pragma unroll
+      for (int r = 0; r < RPW; ++r)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lv[s][sc][r][v] = quantize<S>(out[r][v]);
+    } else {
+      if (p >= zo0 && p < zo1) store_plane(p, out);
+    }
+  }
+
+  template <int PH, int s>
+  __device__ __forceinline__ void stage(int i) {
+    const int p = i - s;
+    if (p < lo(s) || p > hi(s)) return;
+    if (edge_xy || p == 0 || p == geo.nz - 1) stage_impl<PH, s, true>(p);
+    else stage_impl<PH, s, false>(p);
+  }
+
+  template <int PH, int s>
+  __device__ __forceinline__ void stages(int i) {
+    stage<PH, s>(i);
+    if constexpr (s < T) stages<PH, s + 1>(i);
+  }
+
+  template <int PH>
+  __device__ __forceinline__ void iter(int i) {
+    if (i + 1 >= zl0 && i + 1 < zl1) load_plane<mod4(PH + 1)>(i + 1);
+    if constexpr (NW > 1) {
+      if constexpr (X::kBufs == 1) __syncthreads();  // previous plane's reads are done
+      publish<PH>();
+      __syncthreads();
+    }
+    stages<PH, 1>(i);
+  }
+
+  __device__ __forceinline__ void run() {
+    const int i0 = (zl0 - 1) & ~3;  // floor to a multiple of 4 (also for -1)
+    const int ilast = zo1 - 1 + T;
+    for (int i = i0; i <= ilast; i += 4) {
+      iter<0>(i);
+      iter<1>(i + 1);
+      iter<2>(i + 2);
+      iter<3>(i + 3);
+    }
+  }
+};
+
+// Map the hardware workgroup id to a tile so that the workgroups one XCD runs
+// (ids xcd, xcd+8, xcd+16, ...) get consecutive tiles, i.e. spatial neighbours
+// whose shared halo rows stay in that XCD's L2.
+__device__ __forceinline__ int xcd_logical_block(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, xcd = b & 7, slot = b >> 3;
+  return xcd * q + min(xcd, r) + slot;
+}
+
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+__global__ void __launch_bounds__(64 * NW)
+stencil_kernel(const typename Traits<S>::T *__restrict__ src, typename Traits<S>::T *__restrict__ dst,
+               Geometry geo, Coef<typename Traits<S>::C> k) {
+  using R = TileRunner<S, VEC, T, G, RPW, NW, FMA>;
+  __shared__ typename Traits<S>::C xch[R::X::kElems];
+  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+  const int tx = lb % geo.ntx;
+  const int rest = lb / geo.ntx;
+  const int ty = rest % geo.nty;
+  const int tz = rest / geo.nty;
+  R tr(src, dst, geo, k, xch, tx, ty, tz);
+  tr.run();
+}
+
+}  // namespace d3h

@@ -1,0 +1,214 @@
+"""Python host API over the C ABI: contexts, device-resident fields, batches,
+servers and the host-side analytical field.
+
+Arrays cross the boundary as dense numpy arrays in (z, y, x) order, i.e. the
+reference's OFFSET3D layout (x fastest, diffusion3d/diffusion3d.h:33-34).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from ._native import check, lib
+
+NP_DTYPES = {N.F32: np.float32, N.F64: np.float64, N.BF16: np.uint16, N.F16: np.float16}
+DTYPE_NAMES = {N.F32: "f32", N.F64: "f64", N.BF16: "bf16", N.F16: "f16"}
+DTYPE_BY_NAME = {v: k for k, v in DTYPE_NAMES.items()}
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _coef_array(coef):
+    c = (ctypes.c_double * 7)(*[float(v) for v in coef])
+    return c
+
+
+# ---- host-side benchmark constants and analytical field ----------------------
+
+def constants_f32(nx, ny, nz):
+    """kappa, dx, dy, dz, kx, ky, kz, dt, ce, cw, cn, cs, ct, cb, cc as float32
+    (diffusion3d/diffusion3d.h:54-66)."""
+    out = np.zeros(15, dtype=np.float32)
+    check(lib().d3h_constants_f32(nx, ny, nz, out.ctypes.data_as(N.P(ctypes.c_float))),
+          "d3h_constants_f32")
+    return out
+
+
+def constants_f64(nx, ny, nz):
+    out = np.zeros(15, dtype=np.float64)
+    check(lib().d3h_constants_f64(nx, ny, nz, out.ctypes.data_as(N.P(ctypes.c_double))),
+          "d3h_constants_f64")
+    return out
+
+
+def coefficients(constants):
+    """The 7 stencil coefficients ce cw cn cs ct cb cc of a constants vector."""
+    return [float(v) for v in constants[8:15]]
+
+
+def initialize_f32(nx, ny, nz, constants, time=0.0, out=None):
+    """Cosine-product field at `time` (diffusion3d/diffusion3d.cc:14-39)."""
+    if out is None:
+        out = np.empty((nz, ny, nx), dtype=np.float32)
+    k = np.ascontiguousarray(constants, dtype=np.float32)
+    check(lib().d3h_initialize_f32(out.ctypes.data_as(N.P(ctypes.c_float)), nx, ny, nz,
+                                   k.ctypes.data_as(N.P(ctypes.c_float)), float(time)),
+          "d3h_initialize_f32")
+    return out
+
+
+def initialize_f64(nx, ny, nz, constants, time=0.0, out=None):
+    if out is None:
+        out = np.empty((nz, ny, nx), dtype=np.float64)
+    k = np.ascontiguousarray(constants, dtype=np.float64)
+    check(lib().d3h_initialize_f64(out.ctypes.data_as(N.P(ctypes.c_double)), nx, ny, nz,
+                                   k.ctypes.data_as(N.P(ctypes.c_double)), float(time)),
+          "d3h_initialize_f64")
+    return out
+
+
+def add_noise_f32(buf, seed, amp):
+    check(lib().d3h_add_noise_f32(buf.ctypes.data_as(N.P(ctypes.c_float)), buf.size, seed,
+                                  float(amp)), "d3h_add_noise_f32")
+    return buf
+
+
+def tiles():
+    """The compiled kernel/tile table."""
+    out = []
+    for i in range(lib().d3h_tile_count()):
+        d = N.TileDesc()
+        check(lib().d3h_tile_info(i, ctypes.byref(d)), "d3h_tile_info")
+        out.append({k: getattr(d, k) for k, _ in d._fields_} | {"index": i})
+    return out
+
+
+class Context:
+    """One device context (compute + copy streams, events, tuned tile table)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().d3h_context_create(device, ctypes.byref(h)), "d3h_context_create")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().d3h_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self):
+        cus, fr, tot, lds = ctypes.c_int(), ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+        check(lib().d3h_device_info(self.handle, ctypes.byref(cus), ctypes.byref(fr),
+                                    ctypes.byref(tot), ctypes.byref(lds)), "d3h_device_info")
+        return {"num_cus": cus.value, "free_bytes": fr.value, "total_bytes": tot.value,
+                "lds_bytes": lds.value}
+
+    def field(self, nx, ny, nz, dtype=N.F32):
+        return Field(self, nx, ny, nz, dtype)
+
+    def autotune(self, nx, ny, nz, dtype=N.F32, depth=1, arith=N.ARITH_EXACT):
+        t, z, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().d3h_autotune(self.handle, nx, ny, nz, dtype, depth, arith, ctypes.byref(t),
+                                 ctypes.byref(z), ctypes.byref(ms)), "d3h_autotune")
+        return {"tile": t.value, "zchunks": z.value, "ms_per_step": ms.value}
+
+    def save_tuning(self, path):
+        check(lib().d3h_autotune_save(self.handle, path.encode()), "d3h_autotune_save")
+
+    def load_tuning(self, path):
+        check(lib().d3h_autotune_load(self.handle, path.encode()), "d3h_autotune_load")
+
+    def bandwidth_probe(self, n, dtype=N.F32, reps=20):
+        g, ms = ctypes.c_double(), ctypes.c_double()
+        check(lib().d3h_bandwidth_probe(self.handle, n, dtype, reps, ctypes.byref(g),
+                                        ctypes.byref(ms)), "d3h_bandwidth_probe")
+        return {"gbps": g.value, "avg_ms": ms.value}
+
+
+class Field:
+    """A device-resident nx*ny*nz field (two ping-pong buffers)."""
+
+    def __init__(self, ctx, nx, ny, nz, dtype=N.F32):
+        h = ctypes.c_void_p()
+        check(lib().d3h_field_create(ctx.handle, nx, ny, nz, dtype, ctypes.byref(h)),
+              "d3h_field_create")
+        self.ctx, self.handle = ctx, h
+        self.nx, self.ny, self.nz, self.dtype = nx, ny, nz, dtype
+
+    def close(self):
+        if self.handle:
+            lib().d3h_field_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, host, host_dtype=None):
+        if host_dtype is None:
+            host_dtype = {np.dtype(np.float32): N.F32, np.dtype(np.float64): N.F64,
+                          np.dtype(np.float16): N.F16, np.dtype(np.uint16): N.BF16}[host.dtype]
+        host = np.ascontiguousarray(host)
+        if host.size != self.nx * self.ny * self.nz:
+            raise ValueError("host array has %d cells, field %d" % (host.size,
+                                                                     self.nx * self.ny * self.nz))
+        if host.dtype != np.dtype(NP_DTYPES[host_dtype]):
+            raise ValueError("host array dtype %s does not match %s" % (host.dtype,
+                                                                         DTYPE_NAMES[host_dtype]))
+        check(lib().d3h_field_upload(self.handle, _dptr(host), host_dtype), "d3h_field_upload")
+
+    def download(self, host_dtype=None):
+        if host_dtype is None:
+            host_dtype = self.dtype
+        out = np.empty((self.nz, self.ny, self.nx), dtype=NP_DTYPES[host_dtype])
+        check(lib().d3h_field_download(self.handle, _dptr(out), host_dtype), "d3h_field_download")
+        return out
+
+    def run(self, count, coef, depth=1, arith=N.ARITH_EXACT, tile=-1, zchunks=0, graph=False):
+        cfg = N.RunConfig(depth, arith, tile, zchunks, 1 if graph else 0)
+        t = N.Timing()
+        check(lib().d3h_field_run(self.handle, count, _coef_array(coef), ctypes.byref(cfg),
+                                  ctypes.byref(t)), "d3h_field_run")
+        return t.as_dict()
+
+    def compare(self, other):
+        out = (ctypes.c_double * 3)()
+        check(lib().d3h_field_compare(self.handle, other.handle, out), "d3h_field_compare")
+        return {"rel_l2": out[0], "max_abs": out[1], "l2": out[2]}
+
+    def error_analytical(self, kappa, time, k, d):
+        out = (ctypes.c_double * 2)()
+        kk = (ctypes.c_double * 3)(*k)
+        dd = (ctypes.c_double * 3)(*d)
+        check(lib().d3h_field_error_analytical(self.handle, kappa, time, kk, dd, out),
+              "d3h_field_error_analytical")
+        return {"rms": out[0], "max_abs": out[1]}
+
+
+def run_kernel(count, f1, coef):
+    """The Physis-shaped drop-in sequence on a float32 (nz, ny, nx) array, in place:
+    initialize_hip / initialize_benchmark_hip / run_kernel_hip / finalize_benchmark_hip.
+    Returns the kernel-only seconds."""
+    nz, ny, nx = f1.shape
+    L = lib()
+    check(L.initialize_hip(0, None, nx, ny, nz), "initialize_hip")
+    try:
+        check(L.initialize_benchmark_hip(nx, ny, nz), "initialize_benchmark_hip")
+        check(L.run_kernel_hip(count, f1.ctypes.data_as(N.P(ctypes.c_float)), nx, ny, nz,
+                               *[float(c) for c in coef]), "run_kernel_hip")
+        s = ctypes.c_float()
+        check(L.kernel_time_hip(ctypes.byref(s)), "kernel_time_hip")
+        return s.value
+    finally:
+        L.finalize_benchmark_hip()

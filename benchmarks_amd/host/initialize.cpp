@@ -1,0 +1,126 @@
+// Host side of the benchmark class: the constructor's constants and the
+// analytical cosine-product field, computed exactly as the reference's host
+// code does so inputs handed to the device are byte-identical to its CPU path.
+//
+//   constants  : diffusion3d/diffusion3d.h:54-66   (REAL = float; dt in double)
+//   Initialize : diffusion3d/diffusion3d.cc:14-39  (float cosf/expf arguments,
+//                double products, x = d * (REAL)(j + 0.5))
+//
+// Built with g++ -ffp-contract=off so no multiply-add is fused, like the
+// reference's g++ -O3 build for baseline x86-64.
+#include <math.h>
+#include <stdint.h>
+
+#include "diffusion3d_hip.h"
+
+extern "C" {
+
+int d3h_constants_f32(int nx, int ny, int nz, float o[15]) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !o) return D3H_ERR_INVALID;
+  const float kappa = (float)0.1;
+  const float l = 1.0f;
+  const float dx = l / (float)nx, dy = l / (float)ny, dz = l / (float)nz;
+  const float kx = (float)(2.0 * M_PI);
+  const float dt = (float)(0.1 * (double)dx * (double)dx / (double)kappa);
+  const float ce = kappa * dt / (dx * dx);
+  const float cn = kappa * dt / (dy * dy);
+  const float ct = kappa * dt / (dz * dz);
+  // (ce + cw + cn + cs + ct + cb) summed in float, subtracted from 1.0 in double
+  float sum = ce + ce;
+  sum = sum + cn;
+  sum = sum + cn;
+  sum = sum + ct;
+  sum = sum + ct;
+  const float cc = (float)(1.0 - (double)sum);
+  const float v[15] = {kappa, dx, dy, dz, kx, kx, kx, dt, ce, ce, cn, cn, ct, ct, cc};
+  for (int i = 0; i < 15; ++i) o[i] = v[i];
+  return D3H_OK;
+}
+
+int d3h_constants_f64(int nx, int ny, int nz, double o[15]) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !o) return D3H_ERR_INVALID;
+  const double kappa = 0.1, l = 1.0;
+  const double dx = l / nx, dy = l / ny, dz = l / nz;
+  const double kx = 2.0 * M_PI;
+  const double dt = 0.1 * dx * dx / kappa;
+  const double ce = kappa * dt / (dx * dx);
+  const double cn = kappa * dt / (dy * dy);
+  const double ct = kappa * dt / (dz * dz);
+  const double cc = 1.0 - (ce + ce + cn + cn + ct + ct);
+  const double v[15] = {kappa, dx, dy, dz, kx, kx, kx, dt, ce, ce, cn, cn, ct, ct, cc};
+  for (int i = 0; i < 15; ++i) o[i] = v[i];
+  return D3H_OK;
+}
+
+int d3h_initialize_f32(float *buf, int nx, int ny, int nz, const float k[15], float time) {
+  if (!buf || !k || nx <= 0 || ny <= 0 || nz <= 0) return D3H_ERR_INVALID;
+  const float kappa = k[0], dx = k[1], dy = k[2], dz = k[3];
+  const float kx = k[4], ky = k[5], kz = k[6];
+  const float ax = expf(-kappa * time * (kx * kx));
+  const float ay = expf(-kappa * time * (ky * ky));
+  const float az = expf(-kappa * time * (kz * kz));
+#pragma omp parallel for schedule(static)
+  for (int jz = 0; jz < nz; jz++) {
+    const float z = dz * (float)((double)jz + 0.5);
+    const double fz = 1.0 - (double)(az * cosf(kz * z));
+    for (int jy = 0; jy < ny; jy++) {
+      const float y = dy * (float)((double)jy + 0.5);
+      const double fy = 1.0 - (double)(ay * cosf(ky * y));
+      float *row = buf + ((size_t)jz * ny + jy) * nx;
+      for (int jx = 0; jx < nx; jx++) {
+        const float x = dx * (float)((double)jx + 0.5);
+        // ((0.125 * fx) * fy) * fz in double, stored to float
+        double f0 = (double)0.125f * (1.0 - (double)(ax * cosf(kx * x)));
+        f0 = f0 * fy;
+        f0 = f0 * fz;
+        row[jx] = (float)f0;
+      }
+    }
+  }
+  return D3H_OK;
+}
+
+int d3h_initialize_f64(double *buf, int nx, int ny, int nz, const double k[15], double time) {
+  if (!buf || !k || nx <= 0 || ny <= 0 || nz <= 0) return D3H_ERR_INVALID;
+  const double kappa = k[0], dx = k[1], dy = k[2], dz = k[3];
+  const double kx = k[4], ky = k[5], kz = k[6];
+  const double ax = exp(-kappa * time * (kx * kx));
+  const double ay = exp(-kappa * time * (ky * ky));
+  const double az = exp(-kappa * time * (kz * kz));
+#pragma omp parallel for schedule(static)
+  for (int jz = 0; jz < nz; jz++) {
+    const double fz = 1.0 - az * cos(kz * (dz * ((double)jz + 0.5)));
+    for (int jy = 0; jy < ny; jy++) {
+      const double fy = 1.0 - ay * cos(ky * (dy * ((double)jy + 0.5)));
+      double *row = buf + ((size_t)jz * ny + jy) * nx;
+      for (int jx = 0; jx < nx; jx++) {
+        double f0 = 0.125 * (1.0 - ax * cos(kx * (dx * ((double)jx + 0.5))));
+        f0 = f0 * fy;
+        f0 = f0 * fz;
+        row[jx] = f0;
+      }
+    }
+  }
+  return D3H_OK;
+}
+
+// Counter-based noise: SplitMix64 finaliser of seed * golden + index + 1; the
+// top 24 bits give u in [0, 1) exactly; noise = amp * (2u - 1) added in float.
+static inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int d3h_add_noise_f32(float *buf, size_t n, uint64_t seed, float amp) {
+  if (!buf) return D3H_ERR_INVALID;
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    const uint64_t h = mix64(seed * 0x9E3779B97F4A7C15ull + (uint64_t)i + 1ull);
+    const float u = (float)(uint32_t)(h >> 40) * 0x1p-24f;
+    buf[i] = buf[i] + amp * (2.0f * u - 1.0f);
+  }
+  return D3H_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+/*
+ * diffusion3d_hip.h -- C ABI of the MI355X (gfx950) 3D 7-point diffusion engine.
+ *
+ * Plain C types only (no HIP or torch types), every entry point returns an int
+ * status (D3H_OK == 0) and never asserts, exits or throws; d3h_last_error()
+ * gives the calling thread's message for the last failure.
+ *
+ * Data layout seen by callers: a dense nx*ny*nz array, x fastest,
+ * OFFSET3D(i,j,k) = i + j*nx + k*nx*ny (diffusion3d/diffusion3d.h:33-34 of the
+ * reference).  Boundaries are clamped by index (diffusion3d/baseline.cc:33-38).
+ * Coefficients are always given in the reference's run_kernel order
+ * ce, cw, cn, cs, ct, cb, cc (diffusion3d/diffusion3d_physis.h:13-15) with
+ * cn applied to y-1 and cs to y+1 (diffusion3d/baseline.cc:35-40).
+ */
+#ifndef DIFFUSION3D_HIP_H_
+#define DIFFUSION3D_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+  D3H_OK = 0,
+  D3H_ERR_INVALID = 1,      /* bad argument (dims, count, dtype, NULL, ...)   */
+  D3H_ERR_HIP = 2,          /* a HIP runtime call failed                      */
+  D3H_ERR_NOMEM = 3,        /* device or pinned host allocation failed        */
+  D3H_ERR_UNSUPPORTED = 4,  /* no kernel for this dtype / depth / tile        */
+  D3H_ERR_STATE = 5,        /* call out of order (not initialised, busy, ...) */
+  D3H_ERR_NOT_FOUND = 6,    /* unknown request / handle id                    */
+  D3H_ERR_UNSTABLE = 7      /* coefficients violate cc >= 0 (scheme unstable) */
+};
+
+/* Storage types.  FP32 and FP64 compute in their own precision; BF16 and
+ * FP16 are storage formats with FP32 arithmetic and round-to-nearest-even. */
+enum { D3H_F32 = 0, D3H_F64 = 1, D3H_BF16 = 2, D3H_F16 = 3 };
+
+/* Arithmetic modes.  EXACT: the reference's 7-term expression with every
+ * multiply and add rounded separately, in the reference's order -- bitwise
+ * equal to the reference CPU build.  FMA: the same expression contracted the
+ * way a contracting compiler does it, fma(cc,c,cw*w) then a chain of fmas. */
+enum { D3H_ARITH_EXACT = 0, D3H_ARITH_FMA = 1 };
+
+/* ---- drop-in variant ABI ---------------------------------------------------
+ * The four Physis entry points of the reference
+ * (diffusion3d/diffusion3d_physis.h:7-17), same argument meaning, plus a
+ * kernel-only timer for DisplayResult's "Kernel-only performance" block
+ * (diffusion3d/diffusion3d_cuda.cu:373-384).  run_kernel_hip advances f1_host
+ * by exactly `count` steps (H2D, launches, D2H; synchronous), like
+ * Diffusion3DCUDA::RunKernel (diffusion3d/diffusion3d_cuda.cu:353-371). */
+int initialize_hip(int argc, char **argv, int nx, int ny, int nz);
+int initialize_benchmark_hip(int nx, int ny, int nz);
+int run_kernel_hip(int count, float *f1_host, int nx, int ny, int nz,
+                   float ce, float cw, float cn, float cs,
+                   float ct, float cb, float cc);
+int finalize_benchmark_hip(void);
+/* Kernel-only seconds of the last run_kernel_hip (events around the launches). */
+int kernel_time_hip(float *seconds);
+
+/* ---- errors ---------------------------------------------------------------- */
+const char *d3h_status_string(int status);
+const char *d3h_last_error(void);
+
+/* ---- context ---------------------------------------------------------------
+ * One context per host thread and device (the reference is single-threaded on
+ * the default stream).  The context owns a compute stream, a copy stream,
+ * events, the autotune table and scratch for reductions. */
+typedef struct d3h_context d3h_context;
+int d3h_context_create(int device, d3h_context **out);
+int d3h_context_destroy(d3h_context *ctx);
+/* Device facts: CU count, free and total HBM bytes, LDS bytes per workgroup. */
+int d3h_device_info(d3h_context *ctx, int *num_cus, size_t *free_bytes,
+                    size_t *total_bytes, int *lds_bytes);
+
+/* Pinned host memory (hipHostMalloc).  The reference pins with cudaMallocHost
+ * (diffusion3d/diffusion3d_cuda.cu:331). */
+int d3h_host_alloc(size_t bytes, void **ptr);
+int d3h_host_free(void *ptr);
+
+/* ---- device-resident fields --------------------------------------------------
+ * A field owns two device buffers (ping-pong), rows padded to 16 bytes. */
+typedef struct d3h_field d3h_field;
+
+typedef struct {
+  int temporal_depth;  /* steps fused per launch: 1 (plain), 2, 4 or 8       */
+  int arith;           /* D3H_ARITH_EXACT or D3H_ARITH_FMA                     */
+  int tile;            /* kernel/tile index (d3h_tile_info); -1 = tuned choice */
+  int zchunks;         /* z-chunks per launch; 0 = the tuned / default value  */
+  int use_graph;       /* 1: capture the launch sequence in a hipGraph         */
+} d3h_run_config;
+
+typedef struct {
+  double kernel_ms;       /* event time around the launch sequence            */
+  double wall_ms;         /* host wall clock of the whole call                */
+  double avg_launch_ms;   /* kernel_ms / launches                             */
+  int launches;           /* kernel launches issued                           */
+  int steps;              /* steps advanced (== count)                        */
+  int temporal_depth;     /* depth used by the fused launches                 */
+  int tile;               /* kernel/tile index used                           */
+  int zchunks;            /* z-chunks used                                    */
+  int blocks;             /* workgroups per launch                            */
+} d3h_timing;
+
+void d3h_run_config_default(d3h_run_config *cfg);
+
+int d3h_field_create(d3h_context *ctx, int nx, int ny, int nz, int dtype,
+                     d3h_field **out);
+int d3h_field_destroy(d3h_field *f);
+int d3h_field_dims(const d3h_field *f, int *nx, int *ny, int *nz, int *dtype);
+/* Copy a dense host array in; host_dtype may differ from the field's dtype
+ * (converted on the device, round-to-nearest-even when narrowing). */
+int d3h_field_upload(d3h_field *f, const void *host, int host_dtype);
+int d3h_field_download(d3h_field *f, void *host, int host_dtype);
+/* Advance the field by exactly `count` steps on the device (synchronous). */
+int d3h_field_run(d3h_field *f, int count, const double coef[7],
+                  const d3h_run_config *cfg, d3h_timing *timing);
+/* FP64 error norms on the device: out[0] = ||a-b||_2 / ||b||_2,
+ * out[1] = max |a-b|, out[2] = ||a-b||_2.  Same dims required. */
+int d3h_field_compare(const d3h_field *a, const d3h_field *b, double out[3]);
+/* RMS (FP64 accumulation) of the field against the analytical cosine product
+ * at time t, evaluated in FP64 on the device:
+ * 0.125 * prod_i (1 - exp(-kappa t k_i^2) cos(k_i x_i)), x_i = d_i (j + 0.5)
+ * (diffusion3d/diffusion3d.cc:14-39).  out[0] = RMS, out[1] = max abs. */
+int d3h_field_error_analytical(const d3h_field *f, double kappa, double time,
+                               const double k[3], const double d[3], double out[2]);
+
+/* Kernel/tile table (the tuning space of diffusion3d/build.py:6-34, searched
+ * at run time).  Returns D3H_ERR_NOT_FOUND past the end. */
+typedef struct {
+  int dtype, temporal_depth, arith;
+  int groups;          /* 16-byte vectors per lane along x                  */
+  int rows_per_wave;   /* rows each wavefront owns                          */
+  int waves;           /* wavefronts per workgroup                          */
+  int tile_x, tile_y;  /* loaded tile extent in cells / rows               */
+} d3h_tile_desc;
+int d3h_tile_count(void);
+int d3h_tile_info(int index, d3h_tile_desc *out);
+
+/* Autotuner: time every matching tile (and z-chunk count) on a scratch field of
+ * the given shape, remember the fastest for that size class; the table can be
+ * saved to / loaded from a text file.  best_tile / best_zchunks may be NULL. */
+int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype,
+                 int temporal_depth, int arith, int *best_tile, int *best_zchunks,
+                 double *best_ms_per_step);
+int d3h_autotune_save(d3h_context *ctx, const char *path);
+int d3h_autotune_load(d3h_context *ctx, const char *path);
+
+/* ---- bandwidth probe ---------------------------------------------------------
+ * Streams f2 = cc * f1 over n elements of `dtype` `reps` times and reports the
+ * average GB/s (read + write bytes), the measured copy roofline
+ * (the reference's "copy" variant, diffusion/diffusion_cuda_copy.cu:8-24). */
+int d3h_bandwidth_probe(d3h_context *ctx, size_t n, int dtype, int reps,
+                        double *gbps, double *avg_ms);
+
+/* ---- host-side analytical field ------------------------------------------------
+ * The constructor's constants and the initial/analytical field of the
+ * reference's benchmark class, computed on the host exactly as the reference
+ * does (diffusion3d/diffusion3d.h:54-66, diffusion3d/diffusion3d.cc:14-39), so
+ * inputs are byte-identical to its CPU path.  constants[15] = kappa, dx, dy,
+ * dz, kx, ky, kz, dt, ce, cw, cn, cs, ct, cb, cc. */
+int d3h_constants_f32(int nx, int ny, int nz, float constants[15]);
+int d3h_constants_f64(int nx, int ny, int nz, double constants[15]);
+int d3h_initialize_f32(float *buf, int nx, int ny, int nz,
+                       const float constants[15], float time);
+int d3h_initialize_f64(double *buf, int nx, int ny, int nz,
+                       const double constants[15], double time);
+/* Cosine product plus counter-based noise U(-amp, amp) keyed by (seed, index). */
+int d3h_add_noise_f32(float *buf, size_t n, uint64_t seed, float amp);
+
+/* ---- batched uneven grids ------------------------------------------------------
+ * Many independent problems advanced together: per size class one kernel
+ * launch covers every tile of every member, T steps per launch, and the
+ * whole step sequence is captured once in a hipGraph and replayed. */
+typedef struct d3h_batch d3h_batch;
+typedef struct {
+  int nx, ny, nz;
+  int dtype;        /* D3H_F32 / D3H_BF16 / D3H_F16 / D3H_F64 */
+  int steps;        /* exact number of steps for this problem */
+  double coef[7];   /* ce cw cn cs ct cb cc                    */
+} d3h_problem;
+
+int d3h_batch_create(d3h_context *ctx, const d3h_problem *problems, int count,
+                     d3h_batch **out);
+int d3h_batch_destroy(d3h_batch *b);
+/* Upload problem i's initial field (dense host array of host_dtype). */
+int d3h_batch_upload(d3h_batch *b, int i, const void *host, int host_dtype);
+int d3h_batch_download(d3h_batch *b, int i, void *host, int host_dtype);
+/* Run every problem to its step count.  arith as above; temporal_depth 0 =
+ * tuned per size class; use_graph 1 = capture + replay. */
+int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph,
+                  d3h_timing *timing);
+
+/* ---- serving ----------------------------------------------------------------------
+ * A request server over one device: requests (grid + steps + initial field)
+ * are admitted as device memory and slots allow, kept resident, batched with
+ * whatever else is running (continuous batching at sweep boundaries), and
+ * copied back when done.  Submission and polling are thread-safe. */
+typedef struct d3h_server d3h_server;
+typedef struct {
+  size_t hbm_budget_bytes;  /* device bytes the server may hold; 0 = 90% of free */
+  int max_active;           /* max concurrently resident requests; 0 = 4096     */
+  int arith;                /* D3H_ARITH_EXACT / D3H_ARITH_FMA                   */
+  int temporal_depth;       /* steps per sweep launch; 0 = 2                     */
+  int sweep_steps;          /* steps per scheduling round; 0 = 8                 */
+} d3h_server_config;
+
+int d3h_server_create(d3h_context *ctx, const d3h_server_config *cfg,
+                      d3h_server **out);
+int d3h_server_destroy(d3h_server *s);
+/* host_in: dense initial field of host_dtype; host_out: where the result is
+ * written (dtype out_dtype) when the request completes.  Both must stay valid
+ * until the request is done.  Returns the request id in *id. */
+int d3h_server_submit(d3h_server *s, const d3h_problem *p, const void *host_in,
+                      int host_dtype, void *host_out, int out_dtype, int64_t *id);
+/* done = 1 when finished; latency_ms from submit to result-in-host-memory. */
+int d3h_server_poll(d3h_server *s, int64_t id, int *done, double *latency_ms);
+int d3h_server_wait(d3h_server *s, int64_t id, double *latency_ms);
+int d3h_server_stats(d3h_server *s, int64_t *completed, int64_t *rejected,
+                     int64_t *sweeps, double *busy_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFUSION3D_HIP_H_ */

@@ -1,0 +1,101 @@
+"""CPU-only checks of the C ABI library: it loads, exports every declared
+symbol, its host-side field/constants match the reference bitwise, and
+argument errors are reported as status codes (no GPU call is made)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import benchmarks_amd as d3
+import oracle
+from benchmarks_amd import _native as N
+from conftest import as_grid, shape_key
+
+
+def test_library_exports_every_header_symbol():
+    names = N.header_functions()
+    assert "run_kernel_hip" in names and "d3h_field_run" in names and len(names) >= 40
+    handle = ctypes.CDLL(N.LIB_PATH)
+    missing = [n for n in names if not hasattr(handle, n)]
+    assert not missing, missing
+    # and the Python signature table covers the same set
+    assert sorted(N.SIGNATURES) == names
+
+
+def test_host_constants_match_reference(golden):
+    meta, _ = golden
+    for key, bits in meta["constants"].items():
+        nx, ny, nz = map(int, key.split("x"))
+        assert list(d3.constants_f32(nx, ny, nz).view(np.uint32)) == bits, key
+        np.testing.assert_array_equal(d3.constants_f64(nx, ny, nz), oracle.constants_f64(nx, ny, nz))
+
+
+def test_host_initialize_matches_reference(golden):
+    meta, arrays = golden
+    for shape in [(8, 8, 8), (16, 16, 16), (12, 10, 7), (33, 17, 5), (5, 9, 31)]:
+        k = d3.constants_f32(*shape)
+        f = d3.initialize_f32(*shape, k)
+        ref = as_grid(arrays["out_%s_c0" % shape_key(shape)], shape)
+        assert np.array_equal(f.view(np.uint32), ref.view(np.uint32))
+    for item in meta["analytical"]:
+        shape = tuple(item["shape"])
+        k = d3.constants_f32(*shape)
+        a = d3.initialize_f32(*shape, k, np.float32(item["count"]) * k[7])
+        assert np.array_equal(a.view(np.uint32), as_grid(arrays[item["key"]], shape).view(np.uint32))
+    k64 = d3.constants_f64(40, 30, 20)
+    np.testing.assert_array_equal(d3.initialize_f64(40, 30, 20, k64, 0.01),
+                                  oracle.initialize_f64(40, 30, 20, k64, 0.01))
+
+
+def test_host_initialize_large_matches_oracle():
+    k = d3.constants_f32(128, 128, 128)
+    a = d3.initialize_f32(128, 128, 128, k)
+    b = oracle.initialize_f32(128, 128, 128, k)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_noise_matches_oracle():
+    base = np.linspace(0, 1, 100003, dtype=np.float32)
+    a = d3.add_noise_f32(base.copy(), 3, 1e-3)
+    b = oracle.add_noise_f32(base, 3, 1e-3)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_tile_table():
+    tl = d3.tiles()
+    assert len(tl) > 50
+    for t in tl:
+        assert t["dtype"] in (0, 1, 2, 3) and t["temporal_depth"] in (1, 2, 4, 8)
+        assert t["tile_y"] == t["rows_per_wave"] * t["waves"]
+        assert t["tile_x"] % 64 == 0
+        assert t["tile_y"] > 2 * t["temporal_depth"] or t["temporal_depth"] == 1
+    for dt in (0, 1, 2, 3):
+        for T in (1, 2, 4):
+            for ar in (0, 1):
+                assert any(t["dtype"] == dt and t["temporal_depth"] == T and t["arith"] == ar
+                           for t in tl), (dt, T, ar)
+    with pytest.raises(d3.D3HError):
+        N.check(N.lib().d3h_tile_info(len(tl), ctypes.byref(N.TileDesc())), "tile_info")
+
+
+def test_argument_errors_are_status_codes():
+    L = N.lib()
+    h = ctypes.c_void_p()
+    assert L.d3h_field_create(None, 8, 8, 8, 0, ctypes.byref(h)) == N.D3H_ERR_INVALID
+    assert b"NULL" in L.d3h_last_error()
+    assert L.d3h_field_run(None, 1, None, None, None) == N.D3H_ERR_INVALID
+    assert L.d3h_field_upload(None, None, 0) == N.D3H_ERR_INVALID
+    assert L.run_kernel_hip(1, None, 8, 8, 8, *([0.1] * 7)) == N.D3H_ERR_STATE
+    assert L.initialize_benchmark_hip(8, 8, 8) == N.D3H_ERR_STATE
+    assert L.d3h_constants_f32(0, 8, 8, None) == N.D3H_ERR_INVALID
+    assert L.d3h_status_string(N.D3H_ERR_UNSTABLE) == b"unstable coefficients"
+    assert L.d3h_context_destroy(None) == N.D3H_OK
+
+
+def test_driver_help():
+    assert os.path.exists(N.DRIVER_PATH)
+    r = subprocess.run([N.DRIVER_PATH, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    assert "--count N" in r.stderr and "--size N" in r.stderr and "--dump" in r.stderr

@@ -1,0 +1,210 @@
+"""GPU parity of the plain (one step per launch) kernels, through the C ABI.
+
+EXACT mode must be bitwise equal to the reference CPU path (fixtures from the
+reference's own build) and to the oracle; FMA mode must stay within 1e-6
+relative L2 (FP32).  FP64 is checked against the oracle's FP64 instantiation
+(bitwise in EXACT mode; the stated tolerance is 1e-13 relative L2).
+"""
+import hashlib
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import benchmarks_amd as d3
+import oracle
+from benchmarks_amd import _native as N
+from conftest import as_grid, shape_key
+
+pytestmark = pytest.mark.gpu
+
+FULL_SHAPES = [(8, 8, 8), (16, 16, 16), (12, 10, 7), (33, 17, 5), (5, 9, 31)]
+FP32_FMA_REL_L2 = 1e-6
+FP64_REL_L2 = 1e-13
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def gpu_run(ctx, f0, coef, count, dtype=d3.F32, **kw):
+    nz, ny, nx = f0.shape
+    with d3.Field(ctx, nx, ny, nz, dtype) as f:
+        f.upload(f0)
+        t = f.run(count, coef, **kw)
+        return f.download(), t
+
+
+def test_device(ctx):
+    info = ctx.info()
+    print("device", info, flush=True)
+    assert info["num_cus"] >= 1 and info["total_bytes"] > 0
+
+
+def test_reference_fixtures_bitwise(ctx, golden):
+    meta, arrays = golden
+    for case in meta["cases"]:
+        shape = tuple(case["shape"])
+        k = d3.constants_f32(*shape)
+        f0 = d3.initialize_f32(*shape, k)
+        out, _ = gpu_run(ctx, f0, d3.coefficients(k), case["count"])
+        if shape in FULL_SHAPES:
+            ref = as_grid(arrays["out_%s_c%d" % (shape_key(shape), case["count"])], shape)
+            assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), case
+        else:
+            assert sha(out) == case["sha256"], case
+        acc = oracle.accuracy_f32(out, k, case["count"])
+        assert np.float32(acc).view(np.uint32) == case["accuracy_bits"], case
+
+
+def test_config1_128_cubed(ctx, golden):
+    """128^3 FP32, 100 steps: the reference's CPU-runnable configuration."""
+    meta, _ = golden
+    case = [c for c in meta["cases"] if c["shape"] == [128, 128, 128] and c["count"] == 100][0]
+    k = d3.constants_f32(128, 128, 128)
+    for graph in (False, True):
+        out, _ = gpu_run(ctx, d3.initialize_f32(128, 128, 128, k), d3.coefficients(k), 100,
+                         graph=graph)
+        assert sha(out) == case["sha256"]
+        assert "%e" % oracle.accuracy_f32(out, k, 100) == case["accuracy"] == "7.227031e-07"
+
+
+@pytest.mark.parametrize("shape", [(300, 37, 23), (520, 70, 41), (64, 3, 2), (1, 1, 1), (7, 1, 9)])
+def test_every_plain_tile_exact_matches_oracle(ctx, shape):
+    k = d3.constants_f32(*shape)
+    f0 = d3.initialize_f32(*shape, k)
+    rng = np.random.default_rng(11)
+    f0 = (f0 + rng.standard_normal(f0.shape).astype(np.float32) * np.float32(1e-3))
+    want = oracle.run_f32(f0, k[8:15], 5)
+    for t in d3.tiles():
+        if t["dtype"] != d3.F32 or t["temporal_depth"] != 1 or t["arith"] != d3.ARITH_EXACT:
+            continue
+        for zc in (0, 1, 3):
+            out, _ = gpu_run(ctx, f0, d3.coefficients(k), 5, tile=t["index"], zchunks=zc)
+            assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), (t, zc)
+
+
+def test_fma_mode_within_tolerance(ctx):
+    shape = (128, 128, 128)
+    k = d3.constants_f32(*shape)
+    f0 = d3.initialize_f32(*shape, k)
+    want = oracle.run_f32(f0, k[8:15], 100)
+    out, _ = gpu_run(ctx, f0, d3.coefficients(k), 100, arith=d3.ARITH_FMA)
+    rel, mx, _ = oracle.diff_norms(out, want)
+    print("fma rel_l2 %.3e max %.3e" % (rel, mx))
+    assert 0 < rel <= FP32_FMA_REL_L2
+
+
+def test_fma_mode_reference_default_accuracy(ctx):
+    """256^3/100 with the contracted expression: Accuracy 1.982549e-07 (the
+    reference's CPU build prints 4.336409e-07; run.sh's CUDA gate expects e-07)."""
+    k = d3.constants_f32(256, 256, 256)
+    out, _ = gpu_run(ctx, d3.initialize_f32(256, 256, 256, k), d3.coefficients(k), 100,
+                     arith=d3.ARITH_FMA)
+    assert "%e" % oracle.accuracy_f32(out, k, 100) == "1.982549e-07"
+
+
+@pytest.mark.parametrize("shape,count", [((33, 17, 5), 7), ((64, 64, 64), 50), ((130, 20, 9), 3)])
+def test_f64_exact_matches_oracle(ctx, shape, count):
+    k = d3.constants_f64(*shape)
+    f0 = d3.initialize_f64(*shape, k)
+    want = oracle.run_f64(f0, k[8:15], count)
+    out, _ = gpu_run(ctx, f0, d3.coefficients(k), count, dtype=d3.F64)
+    assert np.array_equal(out, want)
+    out2, _ = gpu_run(ctx, f0, d3.coefficients(k), count, dtype=d3.F64, arith=d3.ARITH_FMA)
+    assert oracle.diff_norms(out2, want)[0] <= FP64_REL_L2
+
+
+def test_every_f64_plain_tile(ctx):
+    shape = (300, 37, 23)
+    k = d3.constants_f64(*shape)
+    f0 = d3.initialize_f64(*shape, k)
+    want = oracle.run_f64(f0, k[8:15], 3)
+    for t in d3.tiles():
+        if t["dtype"] == d3.F64 and t["temporal_depth"] == 1 and t["arith"] == d3.ARITH_EXACT:
+            out, _ = gpu_run(ctx, f0, d3.coefficients(k), 3, dtype=d3.F64, tile=t["index"])
+            assert np.array_equal(out, want), t
+
+
+@pytest.mark.parametrize("kind,dtype", [(0, d3.BF16), (1, d3.F16)])
+def test_low_precision_storage_matches_oracle(ctx, kind, dtype):
+    shape = (200, 45, 17)
+    k = d3.constants_f32(*shape)
+    f32 = d3.initialize_f32(*shape, k)
+    bits = oracle.to_lp(f32, kind)
+    want = oracle.run_lp(bits, k[8:15], 9, kind)
+    for t in d3.tiles():
+        if t["dtype"] != dtype or t["temporal_depth"] != 1 or t["arith"] != d3.ARITH_EXACT:
+            continue
+        nz, ny, nx = shape[::-1]
+        with d3.Field(ctx, shape[0], shape[1], shape[2], dtype) as f:
+            f.upload(f32, d3.F32)          # converted on the device (RNE)
+            f.run(9, d3.coefficients(k), tile=t["index"])
+            got = f.download()
+        got_bits = got.view(np.uint16) if dtype == d3.F16 else got
+        assert np.array_equal(got_bits, want), t
+
+
+def test_graph_replay_matches_direct(ctx):
+    shape = (96, 80, 64)
+    k = d3.constants_f32(*shape)
+    f0 = d3.initialize_f32(*shape, k)
+    want = oracle.run_f32(f0, k[8:15], 40)
+    with d3.Field(ctx, *shape, d3.F32) as f:
+        for rep in range(3):           # first call captures, later calls replay
+            f.upload(f0)
+            f.run(40, d3.coefficients(k), graph=True)
+            assert np.array_equal(f.download().view(np.uint32), want.view(np.uint32)), rep
+        f.upload(f0)
+        f.run(39, d3.coefficients(k), graph=True)   # new key -> recapture, odd count
+        assert np.array_equal(f.download().view(np.uint32),
+                              oracle.run_f32(f0, k[8:15], 39).view(np.uint32))
+
+
+def test_physis_shaped_drop_in(ctx):
+    shape = (64, 48, 40)
+    k = d3.constants_f32(*shape)
+    f = d3.initialize_f32(*shape, k)
+    want = oracle.run_f32(f, k[8:15], 9)
+    secs = d3.run_kernel(9, f, k[8:15])
+    assert secs > 0
+    assert np.array_equal(f.view(np.uint32), want.view(np.uint32))
+
+
+def test_device_error_norms_match_host(ctx):
+    shape = (70, 50, 30)
+    k = d3.constants_f32(*shape)
+    k64 = d3.constants_f64(*shape)
+    f0 = d3.initialize_f32(*shape, k)
+    with d3.Field(ctx, *shape, d3.F32) as a, d3.Field(ctx, *shape, d3.F64) as b:
+        a.upload(f0)
+        b.upload(f0.astype(np.float64))
+        a.run(20, d3.coefficients(k))
+        b.run(20, d3.coefficients(k64))
+        cmp = a.compare(b)
+        ha, hb = a.download(), b.download()
+        rel, mx, l2 = oracle.diff_norms(ha.astype(np.float64), hb)
+        assert abs(cmp["rel_l2"] - rel) <= 1e-12 * max(rel, 1e-30) + 1e-300
+        assert cmp["max_abs"] == mx
+        t = 20 * float(k64[7])
+        e = b.error_analytical(float(k64[0]), t, k64[4:7], k64[1:4])
+        want = oracle.rms_vs_analytical_f64(hb, k64, 20)
+        assert abs(e["rms"] - want) <= 1e-9 * want
+
+
+def test_bandwidth_probe(ctx):
+    r = ctx.bandwidth_probe(1 << 27, d3.F32, reps=10)
+    print("probe", r)
+    assert r["gbps"] > 100
+
+
+def test_driver_matches_reference_output_format():
+    r = subprocess.run([N.DRIVER_PATH, "--size", "128", "--count", "100"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    assert "Running diffusion3d/hip" in out and "Kernel-only performance:" in out
+    acc = re.search(r"Accuracy\s+:\s+(\S+)", out).group(1)
+    assert acc == "7.227031e-07", out
