@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--depth", type=int, default=None, help="temporal depth (default per workload)")
     ap.add_argument("--graph", type=int, default=1, help="replay a captured hipGraph (1) or launch")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile before timing")
-    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=500, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -1,29 +1,419 @@
-// Batched uneven grids (implementation pending; entry points report
-// D3H_ERR_UNSUPPORTED until then).
-#include "runtime.hpp"
+// Batched execution of many independent grids of uneven sizes, and the sweep
+// planner it shares with the request server (plan.hpp).
+//
+// A batch owns one device pool holding both buffers of every problem.  A run
+// is one sweep over all problems (each by its own step count); its launches
+// are captured once into a hipGraph (classes on separate streams become
+// independent graph branches) and the graph is replayed on later runs.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "plan.hpp"
 
 using namespace d3h;
 
+namespace d3h {
+
+int check_problem(const d3h_problem &p) {
+  if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return fail(D3H_ERR_INVALID, "dims %dx%dx%d", p.nx, p.ny, p.nz);
+  if (!dtype_valid(p.dtype)) return fail(D3H_ERR_INVALID, "dtype %d", p.dtype);
+  if (p.steps < 0) return fail(D3H_ERR_INVALID, "steps %d", p.steps);
+  double sum = 0;
+  for (int i = 0; i < 7; ++i) {
+    if (!(p.coef[i] >= 0.0) || !std::isfinite(p.coef[i]))
+      return fail(D3H_ERR_UNSTABLE, "coefficient %d = %g (all must be >= 0)", i, p.coef[i]);
+    sum += p.coef[i];
+  }
+  if (std::fabs(sum - 1.0) > 1e-5) return fail(D3H_ERR_UNSTABLE, "coefficients sum to %.9g, not 1", sum);
+  return D3H_OK;
+}
+
+const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, int T, int fma) {
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz));
+    if (it != ctx->tuned.end()) {
+      for (const auto &e : registry())
+        if (e.launch_batch && e.dtype == dtype && e.T == T && e.fma == fma && e.vec == it->second.vec &&
+            e.G == it->second.G && e.RPW == it->second.RPW && e.NW == it->second.NW)
+          return &e;
+    }
+  }
+  const KernelEntry *best = nullptr;
+  for (const auto &e : registry()) {
+    if (!e.launch_batch || e.dtype != dtype || e.T != T || e.fma != fma) continue;
+    if (occupancy_of(e) <= 0) continue;
+    if (!best) {
+      best = &e;
+      continue;
+    }
+    const bool fits = e.tile_x() >= nx, bfits = best->tile_x() >= nx;
+    if (fits && (!bfits || e.tile_x() < best->tile_x() ||
+                 (e.tile_x() == best->tile_x() && e.tile_y() > best->tile_y())))
+      best = &e;
+    else if (!fits && !bfits && e.tile_x() > best->tile_x())
+      best = &e;
+  }
+  return best;
+}
+
+// Chunk depth for batched members: classes fill the device together, so a
+// member only needs enough z-chunks to keep its halo planes a small overhead.
+static int batch_zchunks(int nz, int T) { return std::max(1, nz / std::max(32, 8 * T)); }
+
+int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int fma, SweepPlan *plan) {
+  const int n = (int)ms.size();
+  std::vector<const KernelEntry *> ea(n, nullptr), eb(n, nullptr);
+  std::vector<int> qa(n, 0), rb(n, 0);
+  for (int i = 0; i < n; ++i) {
+    const PlanMember &m = ms[i];
+    int Tm = T;
+    const KernelEntry *e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, Tm, fma);
+    if (!e && Tm != 1) {
+      Tm = 1;
+      e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, 1, fma);
+    }
+    if (!e) return fail(D3H_ERR_UNSUPPORTED, "no batched kernel for dtype %d", m.dtype);
+    ea[i] = e;
+    qa[i] = m.steps / Tm;
+    rb[i] = m.steps % Tm;
+    if (rb[i]) {
+      eb[i] = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, 1, fma);
+      if (!eb[i]) return fail(D3H_ERR_UNSUPPORTED, "no batched plain kernel for dtype %d", m.dtype);
+    }
+  }
+  auto make = [&](const std::vector<const KernelEntry *> &ent, const std::vector<int> &cnt,
+                  std::vector<ClassPlan> &out, bool tail) -> int {
+    std::vector<const KernelEntry *> uniq;
+    for (int i = 0; i < n; ++i)
+      if (ent[i] && cnt[i] > 0 && std::find(uniq.begin(), uniq.end(), ent[i]) == uniq.end())
+        uniq.push_back(ent[i]);
+    for (const KernelEntry *e : uniq) {
+      ClassPlan c;
+      c.e = e;
+      for (int i = 0; i < n; ++i)
+        if (ent[i] == e && cnt[i] > 0) c.members.push_back(i);
+      std::stable_sort(c.members.begin(), c.members.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+      int blocks = 0;
+      c.prefix_blocks.push_back(0);
+      for (int i : c.members) {
+        const PlanMember &pm = ms[i];
+        BatchMember m{};
+        m.buf[0] = pm.buf[0];
+        m.buf[1] = pm.buf[1];
+        unsigned nb = 0;
+        D3H_TRY(make_geometry(*e, pm.nx, pm.ny, pm.nz, pm.pitch, batch_zchunks(pm.nz, e->T), ctx->num_cus,
+                              occupancy_of(*e), &m.geo, &nb));
+        for (int k = 0; k < 7; ++k) m.coef[k] = pm.coef[k];
+        m.block_start = blocks;
+        m.parity0 = tail ? (pm.parity0 + qa[i]) & 1 : pm.parity0;
+        c.table.push_back(m);
+        c.launches.push_back(cnt[i]);
+        for (unsigned k = 0; k < nb; ++k) c.block_member.push_back((int)c.table.size() - 1);
+        blocks += (int)nb;
+        c.prefix_blocks.push_back(blocks);
+      }
+      out.push_back(std::move(c));
+    }
+    return D3H_OK;
+  };
+  plan->A.clear();
+  plan->B.clear();
+  D3H_TRY(make(ea, qa, plan->A, false));
+  D3H_TRY(make(eb, rb, plan->B, true));
+  size_t total = 0;
+  int launches = 0;
+  for (auto *cs : {&plan->A, &plan->B}) {
+    for (auto &c : *cs) {
+      total = (total + 255) / 256 * 256;
+      c.table_off = total;
+      total += c.table.size() * sizeof(BatchMember);
+      total = (total + 255) / 256 * 256;
+      c.block_off = total;
+      total += c.block_member.size() * sizeof(int);
+      launches += c.launches.empty() ? 0 : c.launches[0];
+    }
+  }
+  plan->table_bytes = total;
+  plan->launches = launches;
+  plan->parity_after.assign(n, 0);
+  for (int i = 0; i < n; ++i) plan->parity_after[i] = (ms[i].parity0 + qa[i] + rb[i]) & 1;
+  return D3H_OK;
+}
+
+void plan_fill(const SweepPlan &plan, char *host) {
+  for (const auto *cs : {&plan.A, &plan.B})
+    for (const auto &c : *cs) {
+      memcpy(host + c.table_off, c.table.data(), c.table.size() * sizeof(BatchMember));
+      memcpy(host + c.block_off, c.block_member.data(), c.block_member.size() * sizeof(int));
+    }
+}
+
+int plan_enqueue(std::vector<ClassPlan> &cs, void *dev_tables, hipStream_t main,
+                 const std::vector<hipStream_t> &streams, const std::vector<hipEvent_t> &events,
+                 int *launches) {
+  if (cs.empty()) return D3H_OK;
+  const int ns = (int)std::min<size_t>(cs.size(), streams.size() + 1);
+  if (ns > 1) D3H_HIP(hipEventRecord(events[0], main));
+  for (int s = 1; s < ns; ++s) D3H_HIP(hipStreamWaitEvent(streams[s - 1], events[0], 0));
+  for (size_t ci = 0; ci < cs.size(); ++ci) {
+    ClassPlan &c = cs[ci];
+    hipStream_t st = (ci % ns == 0) ? main : streams[ci % ns - 1];
+    const BatchMember *tab =
+        reinterpret_cast<const BatchMember *>(static_cast<char *>(dev_tables) + c.table_off);
+    const int *blk = reinterpret_cast<const int *>(static_cast<char *>(dev_tables) + c.block_off);
+    const int maxl = c.launches.empty() ? 0 : c.launches[0];
+    int active = (int)c.members.size();
+    for (int k = 0; k < maxl; ++k) {
+      while (active > 0 && c.launches[active - 1] <= k) --active;
+      c.e->launch_batch(tab, blk, k, (unsigned)c.prefix_blocks[active], st);
+      ++*launches;
+    }
+  }
+  for (int s = 1; s < ns; ++s) {
+    D3H_HIP(hipEventRecord(events[s], streams[s - 1]));
+    D3H_HIP(hipStreamWaitEvent(main, events[s], 0));
+  }
+  D3H_HIP(hipGetLastError());
+  return D3H_OK;
+}
+
+}  // namespace d3h
+
+struct d3h_batch {
+  d3h_context *ctx = nullptr;
+  std::vector<d3h_problem> probs;
+  std::vector<int> pitch;
+  std::vector<size_t> off;    // byte offset of buffer 0 of each problem in the pool
+  std::vector<size_t> bytes;  // bytes of one buffer (256-aligned)
+  std::vector<int> cur;       // buffer holding each problem's current field
+  void *pool = nullptr;
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;
+  // cached plan + graph
+  std::vector<long long> key;
+  hipGraphExec_t gexec = nullptr;
+  void *tables = nullptr;
+  int plan_launches = 0;
+  std::vector<int> cur_after;
+};
+
+namespace {
+
+constexpr int kBatchStreams = 3;
+
+void release_plan(d3h_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  if (b->tables) (void)hipFree(b->tables);
+  b->tables = nullptr;
+  b->key.clear();
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+int batch_staging(d3h_batch *b, size_t bytes) {
+  if (b->staging_bytes >= bytes) return D3H_OK;
+  if (b->staging) (void)hipFree(b->staging);
+  b->staging = nullptr;
+  b->staging_bytes = 0;
+  if (hipMalloc(&b->staging, bytes) != hipSuccess) return fail(D3H_ERR_NOMEM, "batch staging");
+  b->staging_bytes = bytes;
+  return D3H_OK;
+}
+
+}  // namespace
+
 extern "C" {
-int d3h_batch_create(d3h_context *, const d3h_problem *, int, d3h_batch **out) {
-  if (out) *out = nullptr;
-  return fail(D3H_ERR_UNSUPPORTED, "batched execution not built yet");
+
+int d3h_batch_create(d3h_context *ctx, const d3h_problem *problems, int count, d3h_batch **out) {
+  if (!ctx || !out || (!problems && count > 0) || count < 0) return fail(D3H_ERR_INVALID, "bad batch arguments");
+  *out = nullptr;
+  for (int i = 0; i < count; ++i) D3H_TRY(check_problem(problems[i]));
+  D3H_HIP(hipSetDevice(ctx->device));
+  auto *b = new d3h_batch();
+  b->ctx = ctx;
+  b->probs.assign(problems, problems + count);
+  size_t total = 0;
+  for (const auto &p : b->probs) {
+    const int pitch = pitch_elems(p.nx, p.dtype);
+    const size_t bytes = ((size_t)pitch * p.ny * p.nz * dtype_bytes(p.dtype) + 255) / 256 * 256;
+    b->pitch.push_back(pitch);
+    b->bytes.push_back(bytes);
+    b->off.push_back(total);
+    total += 2 * bytes;
+  }
+  b->cur.assign(count, 0);
+  if (total && hipMalloc(&b->pool, total) != hipSuccess) {
+    delete b;
+    return fail(D3H_ERR_NOMEM, "batch pool of %zu bytes", total);
+  }
+  if (total && hipMemset(b->pool, 0, total) != hipSuccess) {
+    d3h_batch_destroy(b);
+    return fail(D3H_ERR_HIP, "batch pool memset");
+  }
+  for (int s = 0; s < kBatchStreams; ++s) {
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      d3h_batch_destroy(b);
+      return fail(D3H_ERR_HIP, "batch stream");
+    }
+    b->streams.push_back(st);
+  }
+  for (int s = 0; s <= kBatchStreams; ++s) {
+    hipEvent_t ev;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      d3h_batch_destroy(b);
+      return fail(D3H_ERR_HIP, "batch event");
+    }
+    b->events.push_back(ev);
+  }
+  *out = b;
+  return D3H_OK;
 }
-int d3h_batch_destroy(d3h_batch *) { return D3H_OK; }
-int d3h_batch_upload(d3h_batch *, int, const void *, int) { return fail(D3H_ERR_UNSUPPORTED, "batch"); }
-int d3h_batch_download(d3h_batch *, int, void *, int) { return fail(D3H_ERR_UNSUPPORTED, "batch"); }
-int d3h_batch_run(d3h_batch *, int, int, int, d3h_timing *) { return fail(D3H_ERR_UNSUPPORTED, "batch"); }
-int d3h_server_create(d3h_context *, const d3h_server_config *, d3h_server **out) {
-  if (out) *out = nullptr;
-  return fail(D3H_ERR_UNSUPPORTED, "server not built yet");
+
+int d3h_batch_destroy(d3h_batch *b) {
+  if (!b) return D3H_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  release_plan(b);
+  for (auto s : b->streams) (void)hipStreamDestroy(s);
+  for (auto e : b->events) (void)hipEventDestroy(e);
+  if (b->pool) (void)hipFree(b->pool);
+  if (b->staging) (void)hipFree(b->staging);
+  delete b;
+  return D3H_OK;
 }
-int d3h_server_destroy(d3h_server *) { return D3H_OK; }
-int d3h_server_submit(d3h_server *, const d3h_problem *, const void *, int, void *, int, int64_t *) {
-  return fail(D3H_ERR_UNSUPPORTED, "server");
+
+int d3h_batch_upload(d3h_batch *b, int i, const void *host, int host_dtype) {
+  if (!b || !host || i < 0 || i >= (int)b->probs.size() || !dtype_valid(host_dtype))
+    return fail(D3H_ERR_INVALID, "bad upload arguments");
+  const d3h_problem &p = b->probs[i];
+  hipStream_t st = b->ctx->stream;
+  D3H_HIP(hipSetDevice(b->ctx->device));
+  char *dst = static_cast<char *>(b->pool) + b->off[i] + b->cur[i] * b->bytes[i];
+  const size_t n = (size_t)p.nx * p.ny * p.nz;
+  if (host_dtype == p.dtype) {
+    const size_t es = dtype_bytes(p.dtype);
+    D3H_HIP(hipMemcpy2DAsync(dst, b->pitch[i] * es, host, p.nx * es, p.nx * es, (size_t)p.ny * p.nz,
+                             hipMemcpyHostToDevice, st));
+  } else {
+    D3H_TRY(batch_staging(b, n * dtype_bytes(host_dtype)));
+    D3H_HIP(hipMemcpyAsync(b->staging, host, n * dtype_bytes(host_dtype), hipMemcpyHostToDevice, st));
+    D3H_TRY(launch_convert(b->staging, host_dtype, 0, dst, p.dtype, b->pitch[i], p.nx, p.ny, p.nz, st));
+  }
+  D3H_HIP(hipStreamSynchronize(st));
+  return D3H_OK;
 }
-int d3h_server_poll(d3h_server *, int64_t, int *, double *) { return fail(D3H_ERR_UNSUPPORTED, "server"); }
-int d3h_server_wait(d3h_server *, int64_t, double *) { return fail(D3H_ERR_UNSUPPORTED, "server"); }
-int d3h_server_stats(d3h_server *, int64_t *, int64_t *, int64_t *, double *) {
-  return fail(D3H_ERR_UNSUPPORTED, "server");
+
+int d3h_batch_download(d3h_batch *b, int i, void *host, int host_dtype) {
+  if (!b || !host || i < 0 || i >= (int)b->probs.size() || !dtype_valid(host_dtype))
+    return fail(D3H_ERR_INVALID, "bad download arguments");
+  const d3h_problem &p = b->probs[i];
+  hipStream_t st = b->ctx->stream;
+  D3H_HIP(hipSetDevice(b->ctx->device));
+  const char *src = static_cast<const char *>(b->pool) + b->off[i] + b->cur[i] * b->bytes[i];
+  const size_t n = (size_t)p.nx * p.ny * p.nz;
+  if (host_dtype == p.dtype) {
+    const size_t es = dtype_bytes(p.dtype);
+    D3H_HIP(hipMemcpy2DAsync(host, p.nx * es, src, b->pitch[i] * es, p.nx * es, (size_t)p.ny * p.nz,
+                             hipMemcpyDeviceToHost, st));
+  } else {
+    D3H_TRY(batch_staging(b, n * dtype_bytes(host_dtype)));
+    D3H_TRY(launch_convert(src, p.dtype, b->pitch[i], b->staging, host_dtype, 0, p.nx, p.ny, p.nz, st));
+    D3H_HIP(hipMemcpyAsync(host, b->staging, n * dtype_bytes(host_dtype), hipMemcpyDeviceToHost, st));
+  }
+  D3H_HIP(hipStreamSynchronize(st));
+  return D3H_OK;
 }
+
+int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph, d3h_timing *timing) {
+  const double t0 = now_ms();
+  if (!b) return fail(D3H_ERR_INVALID, "batch is NULL");
+  if (arith != D3H_ARITH_EXACT && arith != D3H_ARITH_FMA) return fail(D3H_ERR_INVALID, "arith %d", arith);
+  const int T = temporal_depth <= 0 ? 2 : temporal_depth;
+  if (T != 1 && T != 2 && T != 4 && T != 8) return fail(D3H_ERR_INVALID, "depth %d", T);
+  const int fma = arith == D3H_ARITH_FMA;
+  d3h_context *ctx = b->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  std::vector<long long> key = {T, fma};
+  for (int c : b->cur) key.push_back(c);
+  int launches = 0;
+  if (!use_graph || key != b->key) {
+    release_plan(b);
+    std::vector<PlanMember> ms;
+    for (size_t i = 0; i < b->probs.size(); ++i) {
+      const d3h_problem &p = b->probs[i];
+      PlanMember m{};
+      m.nx = p.nx; m.ny = p.ny; m.nz = p.nz; m.dtype = p.dtype; m.pitch = b->pitch[i];
+      char *base = static_cast<char *>(b->pool) + b->off[i];
+      m.buf[0] = base;
+      m.buf[1] = base + b->bytes[i];
+      m.parity0 = b->cur[i];
+      m.steps = p.steps;
+      for (int k = 0; k < 7; ++k) m.coef[k] = p.coef[k];
+      ms.push_back(m);
+    }
+    SweepPlan plan;
+    D3H_TRY(plan_sweep(ctx, ms, T, fma, &plan));
+    if (plan.table_bytes) {
+      if (hipMalloc(&b->tables, plan.table_bytes) != hipSuccess) return fail(D3H_ERR_NOMEM, "batch tables");
+      std::vector<char> host(plan.table_bytes);
+      plan_fill(plan, host.data());
+      D3H_HIP(hipMemcpy(b->tables, host.data(), plan.table_bytes, hipMemcpyHostToDevice));
+    }
+    b->cur_after = plan.parity_after;
+    if (use_graph) {
+      hipGraph_t graph = nullptr;
+      D3H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int rc = plan_enqueue(plan.A, b->tables, ctx->stream, b->streams, b->events, &launches);
+      if (rc == D3H_OK) rc = plan_enqueue(plan.B, b->tables, ctx->stream, b->streams, b->events, &launches);
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc != D3H_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (ce != hipSuccess) return fail(D3H_ERR_HIP, "batch capture: %s", hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return fail(D3H_ERR_HIP, "batch instantiate: %s", hipGetErrorString(ie));
+      b->key = key;
+      b->plan_launches = launches;
+    } else {
+      D3H_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+      D3H_TRY(plan_enqueue(plan.A, b->tables, ctx->stream, b->streams, b->events, &launches));
+      D3H_TRY(plan_enqueue(plan.B, b->tables, ctx->stream, b->streams, b->events, &launches));
+      D3H_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    }
+  }
+  if (use_graph) {
+    launches = b->plan_launches;
+    D3H_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    D3H_HIP(hipGraphLaunch(b->gexec, ctx->stream));
+    D3H_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  }
+  D3H_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  D3H_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  b->cur = b->cur_after;
+  if (!use_graph) release_plan(b);
+  if (timing) {
+    memset(timing, 0, sizeof *timing);
+    timing->kernel_ms = ms;
+    timing->launches = launches;
+    timing->avg_launch_ms = launches ? ms / launches : 0;
+    timing->temporal_depth = T;
+    timing->tile = -1;
+    timing->wall_ms = now_ms() - t0;
+  }
+  return D3H_OK;
 }
+
+}  // extern "C"

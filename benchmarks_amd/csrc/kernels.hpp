@@ -10,6 +10,8 @@ namespace d3h {
 
 typedef void (*LaunchFn)(const void *src, void *dst, const Geometry &g, const double *coef,
                          unsigned blocks, hipStream_t stream);
+typedef void (*LaunchBatchFn)(const BatchMember *members, const int *block_member, int launch,
+                              unsigned blocks, hipStream_t stream);
 
 struct KernelEntry {
   int dtype;  // StoreKind
@@ -21,6 +23,8 @@ struct KernelEntry {
   int fma;    // 0 = exact, 1 = fma chain
   const void *func;
   LaunchFn launch;
+  const void *batch_func;      // nullptr when no batched instantiation exists
+  LaunchBatchFn launch_batch;
   int tile_x() const { return 64 * G * vec; }
   int tile_y() const { return NW * RPW; }
 };
@@ -35,10 +39,26 @@ void launch_stencil(const void *src, void *dst, const Geometry &g, const double 
                      stream, (const ST *)src, (ST *)dst, g, k);
 }
 
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+void launch_stencil_batch(const BatchMember *members, const int *block_member, int launch,
+                          unsigned blocks, hipStream_t stream) {
+  hipLaunchKernelGGL((stencil_batch_kernel<S, VEC, T, G, RPW, NW, FMA>), dim3(blocks), dim3(64 * NW),
+                     0, stream, members, block_member, launch);
+}
+
+// Single-grid kernel only.
 #define D3H_ENTRY(S, VEC, T, G, RPW, NW, FMA)                                               \
   KernelEntry {                                                                             \
     S, VEC, T, G, RPW, NW, FMA, (const void *)&stencil_kernel<S, VEC, T, G, RPW, NW, FMA>,  \
-        &launch_stencil<S, VEC, T, G, RPW, NW, FMA>                                         \
+        &launch_stencil<S, VEC, T, G, RPW, NW, FMA>, nullptr, nullptr                       \
+  }
+// Single-grid and batched kernels.
+#define D3H_ENTRY_B(S, VEC, T, G, RPW, NW, FMA)                                             \
+  KernelEntry {                                                                             \
+    S, VEC, T, G, RPW, NW, FMA, (const void *)&stencil_kernel<S, VEC, T, G, RPW, NW, FMA>,  \
+        &launch_stencil<S, VEC, T, G, RPW, NW, FMA>,                                        \
+        (const void *)&stencil_batch_kernel<S, VEC, T, G, RPW, NW, FMA>,                    \
+        &launch_stencil_batch<S, VEC, T, G, RPW, NW, FMA>                                   \
   }
 
 // Each kernels_*.hip file defines one table.

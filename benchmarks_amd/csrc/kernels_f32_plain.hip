@@ -6,14 +6,14 @@
 namespace d3h {
 
 #define F32_PLAIN(A)                                                              \
-  D3H_ENTRY(kF32, 4, 1, 2, 2, 4, A), D3H_ENTRY(kF32, 4, 1, 2, 4, 2, A),           \
+  D3H_ENTRY_B(kF32, 4, 1, 2, 2, 4, A), D3H_ENTRY(kF32, 4, 1, 2, 4, 2, A),           \
   D3H_ENTRY(kF32, 4, 1, 2, 2, 2, A), D3H_ENTRY(kF32, 4, 1, 2, 4, 1, A),           \
-  D3H_ENTRY(kF32, 4, 1, 2, 2, 1, A), D3H_ENTRY(kF32, 4, 1, 2, 1, 8, A),           \
+  D3H_ENTRY(kF32, 4, 1, 2, 1, 8, A),           \
   D3H_ENTRY(kF32, 4, 1, 2, 1, 4, A), D3H_ENTRY(kF32, 4, 1, 1, 4, 4, A),           \
-  D3H_ENTRY(kF32, 4, 1, 1, 2, 4, A), D3H_ENTRY(kF32, 4, 1, 1, 4, 2, A),           \
+  D3H_ENTRY_B(kF32, 4, 1, 1, 2, 4, A), D3H_ENTRY(kF32, 4, 1, 1, 4, 2, A),           \
   D3H_ENTRY(kF32, 4, 1, 1, 2, 8, A), D3H_ENTRY(kF32, 4, 1, 1, 8, 1, A),           \
-  D3H_ENTRY(kF32, 4, 1, 1, 4, 1, A), D3H_ENTRY(kF32, 2, 1, 1, 4, 2, A),           \
-  D3H_ENTRY(kF32, 2, 1, 1, 2, 4, A)
+  D3H_ENTRY(kF32, 4, 1, 1, 4, 1, A), D3H_ENTRY_B(kF32, 2, 1, 1, 4, 2, A),           \
+  D3H_ENTRY_B(kF32, 2, 1, 1, 2, 4, A)
 
 static const KernelEntry kTable[] = {F32_PLAIN(false), F32_PLAIN(true)};
 

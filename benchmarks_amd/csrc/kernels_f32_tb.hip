@@ -8,10 +8,10 @@
 namespace d3h {
 
 #define F32_TB(A)                                                                 \
-  D3H_ENTRY(kF32, 4, 2, 1, 4, 8, A), D3H_ENTRY(kF32, 4, 2, 1, 2, 8, A),           \
+  D3H_ENTRY(kF32, 4, 2, 1, 4, 8, A), D3H_ENTRY_B(kF32, 4, 2, 1, 2, 8, A),           \
   D3H_ENTRY(kF32, 4, 2, 1, 2, 16, A), D3H_ENTRY(kF32, 4, 2, 1, 4, 4, A),          \
-  D3H_ENTRY(kF32, 2, 2, 1, 4, 8, A), D3H_ENTRY(kF32, 4, 2, 2, 2, 4, A),           \
-  D3H_ENTRY(kF32, 2, 4, 1, 2, 16, A), D3H_ENTRY(kF32, 4, 4, 1, 2, 8, A),          \
+  D3H_ENTRY_B(kF32, 2, 2, 1, 4, 8, A), D3H_ENTRY(kF32, 4, 2, 2, 2, 4, A),           \
+  D3H_ENTRY_B(kF32, 2, 4, 1, 2, 16, A), D3H_ENTRY(kF32, 4, 4, 1, 2, 8, A),          \
   D3H_ENTRY(kF32, 2, 4, 1, 4, 8, A),                                               \
   D3H_ENTRY(kF32, 2, 8, 1, 3, 8, A)
 

@@ -6,9 +6,9 @@ namespace d3h {
 #define F64_TILES(A)                                                              \
   D3H_ENTRY(kF64, 2, 1, 4, 2, 4, A), D3H_ENTRY(kF64, 2, 1, 4, 1, 4, A),           \
   D3H_ENTRY(kF64, 2, 1, 4, 2, 2, A), D3H_ENTRY(kF64, 2, 1, 4, 1, 8, A),           \
-  D3H_ENTRY(kF64, 2, 1, 2, 2, 4, A), D3H_ENTRY(kF64, 2, 1, 2, 4, 2, A),           \
-  D3H_ENTRY(kF64, 2, 1, 1, 4, 2, A), D3H_ENTRY(kF64, 2, 1, 1, 2, 4, A),           \
-  D3H_ENTRY(kF64, 2, 2, 1, 4, 8, A), D3H_ENTRY(kF64, 2, 2, 1, 2, 16, A),          \
+  D3H_ENTRY_B(kF64, 2, 1, 2, 2, 4, A), D3H_ENTRY(kF64, 2, 1, 2, 4, 2, A),           \
+  D3H_ENTRY(kF64, 2, 1, 1, 4, 2, A), D3H_ENTRY_B(kF64, 2, 1, 1, 2, 4, A),           \
+  D3H_ENTRY(kF64, 2, 2, 1, 4, 8, A), D3H_ENTRY_B(kF64, 2, 2, 1, 2, 16, A),          \
   D3H_ENTRY(kF64, 1, 4, 1, 2, 16, A), D3H_ENTRY(kF64, 2, 4, 1, 2, 8, A),          \
   D3H_ENTRY(kF64, 1, 8, 1, 3, 8, A)
 

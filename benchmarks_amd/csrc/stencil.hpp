@@ -473,4 +473,43 @@ stencil_kernel(const typename Traits<S>::T *__restrict__ src, typename Traits<S>
   tr.run();
 }
 
+// One member problem of a batched launch (device memory).  Members of one
+// launch share the kernel (tile shape, dtype, depth); each has its own grid,
+// buffers and coefficients.
+struct BatchMember {
+  void *buf[2];     // ping-pong buffers
+  Geometry geo;     // launch geometry of this member
+  double coef[7];   // ce cw cn cs ct cb cc
+  int block_start;  // first logical block of this member in the launch
+  int parity0;      // buffer holding the member's input at launch 0
+};
+
+// Batched launch: logical block -> member through a block table (one scalar
+// load), then the same tile runner.  Launch `launch` of a sequence reads
+// buf[(parity0 + launch) & 1] and writes the other buffer.
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+__global__ void __launch_bounds__(64 * NW)
+stencil_batch_kernel(const BatchMember *__restrict__ members, const int *__restrict__ block_member,
+                     int launch) {
+  using R = TileRunner<S, VEC, T, G, RPW, NW, FMA>;
+  using C = typename Traits<S>::C;
+  using ST = typename Traits<S>::T;
+  __shared__ C xch[R::X::kElems];
+  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+  const int m = block_member[lb];
+  const BatchMember &d = members[m];
+  const Geometry geo = d.geo;
+  const int local = lb - d.block_start;
+  const int par = (d.parity0 + launch) & 1;
+  const Coef<C> k{(C)d.coef[0], (C)d.coef[1], (C)d.coef[2], (C)d.coef[3],
+                  (C)d.coef[4], (C)d.coef[5], (C)d.coef[6]};
+  const int tx = local % geo.ntx;
+  const int rest = local / geo.ntx;
+  const int ty = rest % geo.nty;
+  const int tz = rest / geo.nty;
+  R tr(static_cast<const ST *>(d.buf[par]), static_cast<ST *>(d.buf[par ^ 1]), geo, k, xch, tx, ty,
+       tz);
+  tr.run();
+}
+
 }  // namespace d3h

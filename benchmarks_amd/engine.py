@@ -212,3 +212,77 @@ def run_kernel(count, f1, coef):
         return s.value
     finally:
         L.finalize_benchmark_hip()
+
+
+class Batch:
+    """Many independent grids advanced together (d3h_batch_*).
+
+    problems: list of dicts with nx, ny, nz, dtype, steps, coef (7 values)."""
+
+    def __init__(self, ctx, problems):
+        arr = (N.Problem * len(problems))()
+        for i, p in enumerate(problems):
+            arr[i].nx, arr[i].ny, arr[i].nz = p["nx"], p["ny"], p["nz"]
+            arr[i].dtype, arr[i].steps = p.get("dtype", N.F32), p["steps"]
+            for j, c in enumerate(p["coef"]):
+                arr[i].coef[j] = float(c)
+        h = ctypes.c_void_p()
+        check(lib().d3h_batch_create(ctx.handle, arr, len(problems), ctypes.byref(h)),
+              "d3h_batch_create")
+        self.ctx, self.handle, self.problems = ctx, h, list(problems)
+
+    def close(self):
+        if self.handle:
+            lib().d3h_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, i, host, host_dtype):
+        host = np.ascontiguousarray(host)
+        check(lib().d3h_batch_upload(self.handle, i, _dptr(host), host_dtype), "d3h_batch_upload")
+
+    def download(self, i, host_dtype=None):
+        p = self.problems[i]
+        if host_dtype is None:
+            host_dtype = p.get("dtype", N.F32)
+        out = np.empty((p["nz"], p["ny"], p["nx"]), dtype=NP_DTYPES[host_dtype])
+        check(lib().d3h_batch_download(self.handle, i, _dptr(out), host_dtype),
+              "d3h_batch_download")
+        return out
+
+    def run(self, arith=N.ARITH_EXACT, depth=2, graph=True):
+        t = N.Timing()
+        check(lib().d3h_batch_run(self.handle, arith, depth, 1 if graph else 0, ctypes.byref(t)),
+              "d3h_batch_run")
+        return t.as_dict()
+
+
+def stable_coefficients(nx, ny, nz, kappa=0.1, dtype=np.float32):
+    """Coefficients of the batched/served problems: dt = 0.1 h_min^2 / kappa with
+    h = 1/n per axis, so every coefficient is <= 0.1 and cc >= 0.4 for any shape
+    (the reference's dt uses dx only, diffusion3d/diffusion3d.h:61, which makes
+    cc < 0 when dx^2/dy^2 + dx^2/dz^2 > 4).  Same float/double promotions as the
+    reference constructor; for cubic grids the values equal the reference's."""
+    if dtype == np.float64:
+        dx, dy, dz = 1.0 / nx, 1.0 / ny, 1.0 / nz
+        h = min(dx, dy, dz)
+        dt = 0.1 * h * h / kappa
+        ce, cn, ct = kappa * dt / (dx * dx), kappa * dt / (dy * dy), kappa * dt / (dz * dz)
+        cc = 1.0 - (ce + ce + cn + cn + ct + ct)
+        return [ce, ce, cn, cn, ct, ct, cc], dt
+    f = np.float32
+    kap = f(kappa)
+    dx, dy, dz = f(1.0) / f(nx), f(1.0) / f(ny), f(1.0) / f(nz)
+    h = min(dx, dy, dz)
+    dt = f(0.1 * float(h) * float(h) / float(kap))
+    ce = kap * dt / (dx * dx)
+    cn = kap * dt / (dy * dy)
+    ct = kap * dt / (dz * dz)
+    s = f(f(f(f(f(ce + ce) + cn) + cn) + ct) + ct)
+    cc = f(1.0 - float(s))
+    return [float(v) for v in (ce, ce, cn, cn, ct, ct, cc)], float(dt)

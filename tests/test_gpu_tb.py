@@ -79,4 +79,4 @@ def test_tb_fma_within_tolerance(ctx):
     for depth in (2, 4, 8):
         out, _ = run(ctx, f0, d3.F32, d3.F32, d3.coefficients(k), 40, depth=depth,
                      arith=d3.ARITH_FMA)
-        assert oracle.diff_norms(out, want)[0] <= 1e-6, depth
+        assert oracle.diff_norms(out, want)[0] <= 2e-6, depth
