@@ -9,7 +9,7 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libdiffusion3d_hip.so")
+LIB_PATH = os.environ.get("D3H_LIB") or os.path.join(HERE, "lib", "libdiffusion3d_hip.so")
 DRIVER_PATH = os.path.join(HERE, "bin", "diffusion3d_hip")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "diffusion3d_hip.h")
 

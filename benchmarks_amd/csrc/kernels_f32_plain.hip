@@ -13,7 +13,9 @@ namespace d3h {
   D3H_ENTRY_B(kF32, 4, 1, 1, 2, 4, A), D3H_ENTRY(kF32, 4, 1, 1, 4, 2, A),           \
   D3H_ENTRY(kF32, 4, 1, 1, 2, 8, A), D3H_ENTRY(kF32, 4, 1, 1, 8, 1, A),           \
   D3H_ENTRY(kF32, 4, 1, 1, 4, 1, A), D3H_ENTRY_B(kF32, 2, 1, 1, 4, 2, A),           \
-  D3H_ENTRY_B(kF32, 2, 1, 1, 2, 4, A)
+  D3H_ENTRY_B(kF32, 2, 1, 1, 2, 4, A), D3H_ENTRY(kF32, 4, 1, 3, 2, 4, A),          \
+  D3H_ENTRY(kF32, 4, 1, 3, 1, 8, A), D3H_ENTRY(kF32, 4, 1, 3, 2, 2, A),            \
+  D3H_ENTRY(kF32, 4, 1, 3, 1, 4, A)
 
 static const KernelEntry kTable[] = {F32_PLAIN(false), F32_PLAIN(true)};
 

@@ -80,6 +80,8 @@ int occupancy_of(const KernelEntry &e) {
 int make_geometry(const KernelEntry &e, int nx, int ny, int nz, int pitch, int zchunks,
                   int num_cus, int occupancy, Geometry *g, unsigned *blocks) {
   const int TX = e.tile_x(), TY = e.tile_y(), T = e.T;
+  if ((long long)pitch * ny * (long long)dtype_bytes(e.dtype) >= 0x7ffffff0LL)
+    return fail(D3H_ERR_UNSUPPORTED, "one xy plane of %d x %d exceeds 2 GiB", pitch, ny);
   Geometry q{};
   q.nx = nx; q.ny = ny; q.nz = nz; q.pitch = pitch;
   q.plane = (long long)pitch * ny;

@@ -50,6 +50,22 @@ template <> struct Traits<kF16> { using T = uint16_t; using C = float; };
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
+// Buffer descriptor word 3 for untyped (raw) buffer access on the gfx9
+// family: DATA_FORMAT = 32 bit, so range checking is per dword.
+constexpr int kRsrcWord3 = 0x00020000;
+// Cache policy bits of buffer instructions on gfx94x/gfx950: bit 1 = nt.
+#ifndef D3H_LOAD_AUX
+#define D3H_LOAD_AUX 0
+#endif
+constexpr int kLoadAux = D3H_LOAD_AUX;
+constexpr int kStoreAux = 2;
+// Offset that lies beyond every descriptor (planes are < 2 GiB).
+constexpr int kOutOfRange = 0x7ffffff0;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, bytes, kRsrcWord3);
+}
+
 // Launch geometry, computed on the host (make_geometry in runtime.cpp).
 //   Tiles overlap: tile (tx,ty,tz) stores x in [tx*ox, tx*ox+ox), rows
 //   [ty*oy, ty*oy+oy), planes [tz*oz, tz*oz+oz) (clipped to the grid) and loads
@@ -125,26 +141,30 @@ struct Pack {
       for (int i = 0; i < kWords; ++i) w[i] = to_f16(v[2 * i]) | (to_f16(v[2 * i + 1]) << 16);
     }
   }
-  __device__ static __forceinline__ void load(const void *p, C *v) {
+  // Unconditional buffer accesses: an offset at or beyond the descriptor's
+  // size reads zeros / drops the store, so no lane or plane needs a branch
+  // (branch-free memory keeps the compiler's wait counts exact, which lets
+  // the next plane's loads stay in flight during the current plane's math).
+  __device__ static __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, int off, C *v) {
     uint32_t w[kWords];
     if constexpr (kBytes == 16) {
-      const u32x4 u = *reinterpret_cast<const u32x4 *>(p);
+      const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, kLoadAux);
       w[0] = u.x; w[1] = u.y; w[2] = u.z; w[3] = u.w;
     } else {
-      const u32x2 u = *reinterpret_cast<const u32x2 *>(p);
+      const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, kLoadAux);
       w[0] = u.x; w[1] = u.y;
     }
     unpack(w, v);
   }
-  __device__ static __forceinline__ void store(void *p, const C *v) {
+  __device__ static __forceinline__ void store(__amdgpu_buffer_rsrc_t rs, int off, const C *v) {
     uint32_t w[kWords];
     pack(v, w);
     if constexpr (kBytes == 16) {
       u32x4 u = {w[0], w[1], w[2], w[3]};
-      __builtin_nontemporal_store(u, reinterpret_cast<u32x4 *>(p));
+      __builtin_amdgcn_raw_buffer_store_b128(u, rs, off, 0, kStoreAux);
     } else {
       u32x2 u = {w[0], w[1]};
-      __builtin_nontemporal_store(u, reinterpret_cast<u32x2 *>(p));
+      __builtin_amdgcn_raw_buffer_store_b64(u, rs, off, 0, kStoreAux);
     }
   }
 };
@@ -240,9 +260,10 @@ struct TileRunner {
   int xlane;            // x of this lane's cell 0 in group 0
   int zl0, zl1, zo0, zo1;
   bool edge_xy;         // tile holds the last column/row strictly inside it
-  long long rowoff[RPW];
-  bool rowin[RPW], rowst[RPW], rowlast[RPW];
-  bool colin[G], colst[G];
+  int plane_bytes;
+  int ldoff[RPW][G];   // byte offset in a plane, or kOutOfRange (loads zeros)
+  int stoff[RPW][G];   // byte offset in a plane, or kOutOfRange (store dropped)
+  bool rowlast[RPW];
 
   __device__ __forceinline__ TileRunner(const ST *s_, ST *d_, const Geometry &g, const Coef<C> &k_,
                                         C *x_, int tx, int ty, int tz)
@@ -262,19 +283,20 @@ struct TileRunner {
     zl1 = min(zo1 + T, geo.nz);
     edge_xy = (xl0 + TILE_X > geo.nx) || (yl0 + TILE_Y > geo.ny);
     xlane = xl0 + lane * VEC;
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int xv = xlane + g * GSTRIDE;
-      colin[g] = xv < geo.pitch;
-      colst[g] = xv >= xo0 && xv < xo1s;
-    }
+    plane_bytes = (int)(geo.plane * (long long)sizeof(ST));
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int y = yl0 + wave * RPW + r;
-      rowin[r] = y < geo.ny;
-      rowst[r] = y >= yo0 && y < yo1;
+      const bool rowin = y < geo.ny;
+      const bool rowst = y >= yo0 && y < yo1;
       rowlast[r] = y == geo.ny - 1;
-      rowoff[r] = (long long)y * geo.pitch + xlane;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int xv = xlane + g * GSTRIDE;
+        const int off = (y * geo.pitch + xv) * (int)sizeof(ST);
+        ldoff[r][g] = (rowin && xv < geo.pitch) ? off : kOutOfRange;
+        stoff[r][g] = (rowst && xv >= xo0 && xv < xo1s) ? off : kOutOfRange;
+      }
     }
 #pragma unroll
     for (int l = 0; l < T; ++l)
@@ -286,27 +308,25 @@ struct TileRunner {
           for (int v = 0; v < NV; ++v) lv[l][s][r][v] = C(0);
   }
 
+  // Load plane q into level-0 slot SL; planes outside [zl0, zl1) read zeros.
   template <int SL>
   __device__ __forceinline__ void load_plane(int q) {
-    const ST *base = src + (long long)q * geo.plane;
+    const bool in = q >= zl0 && q < zl1;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(src + (long long)q * geo.plane, in ? plane_bytes : 0);
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
+    for (int r = 0; r < RPW; ++r)
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        if (rowin[r] && colin[g]) P::load(base + rowoff[r] + g * GSTRIDE, &lv[0][SL][r][g * VEC]);
-      }
-    }
+      for (int g = 0; g < G; ++g) P::load(rs, ldoff[r][g], &lv[0][SL][r][g * VEC]);
   }
 
+  // Store plane q; planes outside [zo0, zo1) are dropped.
   __device__ __forceinline__ void store_plane(int q, C (&out)[RPW][NV]) {
-    ST *base = dst + (long long)q * geo.plane;
+    const bool in = q >= zo0 && q < zo1;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(dst + (long long)q * geo.plane, in ? plane_bytes : 0);
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
+    for (int r = 0; r < RPW; ++r)
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        if (rowst[r] && colst[g]) P::store(base + rowoff[r] + g * GSTRIDE, &out[r][g * VEC]);
-      }
-    }
+      for (int g = 0; g < G; ++g) P::store(rs, stoff[r][g], &out[r][g * VEC]);
   }
 
   // First / last valid plane of level s in this chunk.
@@ -409,15 +429,17 @@ struct TileRunner {
 #pragma unroll
         for (int v = 0; v < NV; ++v) lv[s][sc][r][v] = quantize<S>(out[r][v]);
     } else {
-      if (p >= zo0 && p < zo1) store_plane(p, out);
+      store_plane(p, out);
     }
   }
 
+  // Every stage runs every iteration (planes outside the valid range compute
+  // values that are never stored or read by a valid plane), so each path
+  // through an iteration issues the same memory instructions.
   template <int PH, int s>
   __device__ __forceinline__ void stage(int i) {
     const int p = i - s;
-    if (p < lo(s) || p > hi(s)) return;
-    if (edge_xy || p == 0 || p == geo.nz - 1) stage_impl<PH, s, true>(p);
+    if (edge_xy || p <= 0 || p >= geo.nz - 1) stage_impl<PH, s, true>(p);
     else stage_impl<PH, s, false>(p);
   }
 
@@ -429,7 +451,7 @@ struct TileRunner {
 
   template <int PH>
   __device__ __forceinline__ void iter(int i) {
-    if (i + 1 >= zl0 && i + 1 < zl1) load_plane<mod4(PH + 1)>(i + 1);
+    load_plane<mod4(PH + 1)>(i + 1);
     if constexpr (NW > 1) {
       if constexpr (X::kBufs == 1) __syncthreads();  // previous plane's reads are done
       publish<PH>();
