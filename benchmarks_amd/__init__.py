@@ -4,9 +4,9 @@ Hand-written HIP kernels for gfx950 behind a C ABI (include/diffusion3d_hip.h),
 a C++ benchmark-class driver (benchmarks_amd/host) and this Python binding.
 """
 from ._native import (ARITH_EXACT, ARITH_FMA, BF16, F16, F32, F64, D3HError, LIB_PATH, lib)
-from .engine import (Batch, Context, Field, stable_coefficients, add_noise_f32, coefficients, constants_f32, constants_f64,
+from .engine import (Batch, Context, Field, Server, stable_coefficients, add_noise_f32, coefficients, constants_f32, constants_f64,
                      initialize_f32, initialize_f64, run_kernel, tiles)
 
 __all__ = ["ARITH_EXACT", "ARITH_FMA", "BF16", "F16", "F32", "F64", "D3HError", "LIB_PATH", "lib",
-           "Batch", "Context", "Field", "stable_coefficients", "add_noise_f32", "coefficients", "constants_f32", "constants_f64",
+           "Batch", "Context", "Field", "Server", "stable_coefficients", "add_noise_f32", "coefficients", "constants_f32", "constants_f64",
            "initialize_f32", "initialize_f64", "run_kernel", "tiles"]

@@ -1,21 +1,498 @@
-// Request server (implementation pending; entry points report
-// D3H_ERR_UNSUPPORTED until then).
-#include "runtime.hpp"
+// Request server: continuous batching of diffusion solves on one device.
+//
+//   submit()  -> queue (FIFO)                         [any thread]
+//   scheduler thread, each round:
+//     admit   : pop queued requests while a slot is free and the request's
+//               footprint (two pitched buffers + conversion staging) fits
+//               in the server's device pool (first-fit, coalescing free list)
+//     upload  : H2D (+ dtype conversion) on the copy stream; the compute
+//               stream waits on it before the next sweep
+//     sweep   : every active request advances min(remaining, sweep_steps)
+//               steps; one launch per kernel class covers all of them (the
+//               batched sweep planner), so requests that arrived while others
+//               were running share the next launches
+//     finish  : requests that reach their step count are copied back on the
+//               copy stream after their last sweep; a host callback stamps
+//               the completion time and wakes waiters; the scheduler then
+//               returns their memory to the pool
+//   Two sweeps may be in flight (double-buffered launch tables), so host-side
+//   planning of sweep k+1 overlaps the device running sweep k.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <unordered_map>
+
+#include "plan.hpp"
 
 using namespace d3h;
 
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over [0, size) with coalescing (scheduler thread only).
+class Pool {
+ public:
+  void reset(size_t size) {
+    free_.clear();
+    if (size) free_[0] = size;
+  }
+  bool allocate(size_t bytes, size_t *off) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes;
+        const size_t roff = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[roff] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<size_t, size_t> free_;
+};
+
+enum State { kQueued, kActive, kDraining, kDone, kFailed };
+
+struct Request {
+  int64_t id = 0;
+  d3h_problem p{};
+  const void *in = nullptr;
+  int in_dt = 0;
+  void *out = nullptr;
+  int out_dt = 0;
+  double t_submit = 0, t_done = 0;
+  std::atomic<int> state{kQueued};
+  int status = D3H_OK;
+  size_t off = 0, footprint = 0, buf_bytes = 0, stage_off = 0;
+  int pitch = 0;
+  int cur = 0;
+  int remaining = 0;
+  bool freed = false;
+};
+
+}  // namespace
+
+struct d3h_server {
+  d3h_context *ctx = nullptr;
+  d3h_server_config cfg{};
+  void *pool = nullptr;
+  size_t pool_bytes = 0;
+  Pool alloc;
+  hipStream_t compute = nullptr, copy = nullptr;
+  std::vector<hipStream_t> extra;
+  std::vector<hipEvent_t> fork_events;
+  hipEvent_t upload_ev = nullptr;
+  hipEvent_t sweep_ev[2] = {nullptr, nullptr};
+  bool sweep_pending[2] = {false, false};
+  void *dev_tables[2] = {nullptr, nullptr};
+  char *host_tables[2] = {nullptr, nullptr};
+  size_t table_cap = 0;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  int fatal = D3H_OK;
+  std::deque<Request *> queue;
+  std::vector<Request *> active, draining;
+  std::unordered_map<int64_t, Request *> reqs;
+  int64_t next_id = 1;
+  int64_t completed = 0, rejected = 0, sweeps = 0;
+  double busy_ms = 0;
+};
+
+namespace {
+
+void on_request_done(void *arg) {
+  auto *pair = static_cast<std::pair<d3h_server *, Request *> *>(arg);
+  d3h_server *s = pair->first;
+  Request *r = pair->second;
+  delete pair;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->t_done = now_ms();
+    r->state = kDone;
+    ++s->completed;
+  }
+  s->cv_done.notify_all();
+  s->cv_work.notify_all();
+}
+
+int ensure_tables(d3h_server *s, size_t bytes) {
+  if (bytes <= s->table_cap) return D3H_OK;
+  // no sweep may still read the old tables
+  D3H_HIP(hipStreamSynchronize(s->compute));
+  for (int i = 0; i < 2; ++i) {
+    if (s->dev_tables[i]) (void)hipFree(s->dev_tables[i]);
+    if (s->host_tables[i]) (void)hipHostFree(s->host_tables[i]);
+    s->dev_tables[i] = nullptr;
+    s->host_tables[i] = nullptr;
+    s->sweep_pending[i] = false;
+  }
+  const size_t cap = std::max(bytes * 2, (size_t)1 << 20);
+  for (int i = 0; i < 2; ++i) {
+    if (hipMalloc(&s->dev_tables[i], cap) != hipSuccess) return fail(D3H_ERR_NOMEM, "server tables");
+    if (hipHostMalloc((void **)&s->host_tables[i], cap, hipHostMallocDefault) != hipSuccess)
+      return fail(D3H_ERR_NOMEM, "server pinned tables");
+  }
+  s->table_cap = cap;
+  return D3H_OK;
+}
+
+char *field_ptr(d3h_server *s, Request *r, int which) {
+  return static_cast<char *>(s->pool) + r->off + which * r->buf_bytes;
+}
+
+int upload(d3h_server *s, Request *r) {
+  const d3h_problem &p = r->p;
+  const size_t n = (size_t)p.nx * p.ny * p.nz;
+  char *dst = field_ptr(s, r, 0);
+  if (r->in_dt == p.dtype) {
+    const size_t es = dtype_bytes(p.dtype);
+    D3H_HIP(hipMemcpy2DAsync(dst, r->pitch * es, r->in, p.nx * es, p.nx * es, (size_t)p.ny * p.nz,
+                             hipMemcpyHostToDevice, s->copy));
+  } else {
+    char *stage = static_cast<char *>(s->pool) + r->stage_off;
+    D3H_HIP(hipMemcpyAsync(stage, r->in, n * dtype_bytes(r->in_dt), hipMemcpyHostToDevice, s->copy));
+    D3H_TRY(launch_convert(stage, r->in_dt, 0, dst, p.dtype, r->pitch, p.nx, p.ny, p.nz, s->copy));
+  }
+  r->cur = 0;
+  return D3H_OK;
+}
+
+int download(d3h_server *s, Request *r) {
+  const d3h_problem &p = r->p;
+  const size_t n = (size_t)p.nx * p.ny * p.nz;
+  const char *src = field_ptr(s, r, r->cur);
+  if (r->out_dt == p.dtype) {
+    const size_t es = dtype_bytes(p.dtype);
+    D3H_HIP(hipMemcpy2DAsync(r->out, p.nx * es, src, r->pitch * es, p.nx * es, (size_t)p.ny * p.nz,
+                             hipMemcpyDeviceToHost, s->copy));
+  } else {
+    char *stage = static_cast<char *>(s->pool) + r->stage_off;
+    D3H_TRY(launch_convert(src, p.dtype, r->pitch, stage, r->out_dt, 0, p.nx, p.ny, p.nz, s->copy));
+    D3H_HIP(hipMemcpyAsync(r->out, stage, n * dtype_bytes(r->out_dt), hipMemcpyDeviceToHost, s->copy));
+  }
+  auto *arg = new std::pair<d3h_server *, Request *>(s, r);
+  hipError_t e = hipLaunchHostFunc(s->copy, on_request_done, arg);
+  if (e != hipSuccess) {
+    delete arg;
+    return fail(D3H_ERR_HIP, "hipLaunchHostFunc: %s", hipGetErrorString(e));
+  }
+  return D3H_OK;
+}
+
+// One scheduling round; returns true when it issued device work.
+int round_once(d3h_server *s, bool *did_work) {
+  *did_work = false;
+  std::vector<Request *> admitted;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    // return memory of finished requests
+    for (auto it = s->draining.begin(); it != s->draining.end();) {
+      Request *r = *it;
+      if (r->state.load() == kDone) {
+        s->alloc.release(r->off, r->footprint);
+        r->freed = true;
+        it = s->draining.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    while (!s->queue.empty() && (int)(s->active.size() + admitted.size()) < s->cfg.max_active) {
+      Request *r = s->queue.front();
+      size_t off = 0;
+      if (!s->alloc.allocate(r->footprint, &off)) break;  // FIFO: wait for memory
+      s->queue.pop_front();
+      r->off = off;
+      r->stage_off = off + 2 * r->buf_bytes;
+      r->state = kActive;
+      admitted.push_back(r);
+    }
+  }
+  if (!admitted.empty()) {
+    for (Request *r : admitted) {
+      D3H_TRY(upload(s, r));
+      r->remaining = r->p.steps;
+    }
+    D3H_HIP(hipEventRecord(s->upload_ev, s->copy));
+    D3H_HIP(hipStreamWaitEvent(s->compute, s->upload_ev, 0));
+    s->active.insert(s->active.end(), admitted.begin(), admitted.end());
+    *did_work = true;
+  }
+  std::vector<Request *> members, finishing;
+  for (Request *r : s->active) {
+    if (r->remaining > 0) members.push_back(r);
+    else finishing.push_back(r);  // zero-step requests
+  }
+  if (!members.empty()) {
+    const int slot = (int)(s->sweeps & 1);
+    if (s->sweep_pending[slot]) D3H_HIP(hipEventSynchronize(s->sweep_ev[slot]));
+    std::vector<PlanMember> ms;
+    for (Request *r : members) {
+      PlanMember m{};
+      m.nx = r->p.nx; m.ny = r->p.ny; m.nz = r->p.nz; m.dtype = r->p.dtype; m.pitch = r->pitch;
+      m.buf[0] = field_ptr(s, r, 0);
+      m.buf[1] = field_ptr(s, r, 1);
+      m.parity0 = r->cur;
+      m.steps = std::min(r->remaining, s->cfg.sweep_steps);
+      for (int k = 0; k < 7; ++k) m.coef[k] = r->p.coef[k];
+      ms.push_back(m);
+    }
+    SweepPlan plan;
+    D3H_TRY(plan_sweep(s->ctx, ms, s->cfg.temporal_depth, s->cfg.arith == D3H_ARITH_FMA, &plan));
+    D3H_TRY(ensure_tables(s, plan.table_bytes));
+    plan_fill(plan, s->host_tables[slot]);
+    D3H_HIP(hipMemcpyAsync(s->dev_tables[slot], s->host_tables[slot], plan.table_bytes,
+                           hipMemcpyHostToDevice, s->compute));
+    int launches = 0;
+    D3H_TRY(plan_enqueue(plan.A, s->dev_tables[slot], s->compute, s->extra, s->fork_events, &launches));
+    D3H_TRY(plan_enqueue(plan.B, s->dev_tables[slot], s->compute, s->extra, s->fork_events, &launches));
+    D3H_HIP(hipEventRecord(s->sweep_ev[slot], s->compute));
+    s->sweep_pending[slot] = true;
+    ++s->sweeps;
+    for (size_t i = 0; i < members.size(); ++i) {
+      Request *r = members[i];
+      r->cur = plan.parity_after[i];
+      r->remaining -= ms[i].steps;
+      if (r->remaining == 0) finishing.push_back(r);
+    }
+    // results are copied after this sweep
+    if (!finishing.empty()) D3H_HIP(hipStreamWaitEvent(s->copy, s->sweep_ev[slot], 0));
+    *did_work = true;
+  }
+  if (!finishing.empty()) {
+    for (Request *r : finishing) D3H_TRY(download(s, r));
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Request *r : finishing) {
+      r->state = r->state.load() == kDone ? (int)kDone : (int)kDraining;
+      s->draining.push_back(r);
+      s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+    }
+  }
+  return D3H_OK;
+}
+
+void scheduler(d3h_server *s) {
+  (void)hipSetDevice(s->ctx->device);
+  double busy_start = -1;
+  while (true) {
+    bool work = false;
+    int rc = round_once(s, &work);
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (rc != D3H_OK) {
+      s->fatal = rc;
+      // fail everything that has not completed
+      for (auto &kv : s->reqs) {
+        Request *r = kv.second;
+        const int st = r->state.load();
+        if (st != kDone && st != kFailed) {
+          r->state = kFailed;
+          r->status = rc;
+        }
+      }
+      s->queue.clear();
+      s->cv_done.notify_all();
+      return;
+    }
+    const bool idle = s->queue.empty() && s->active.empty();
+    if (!idle && busy_start < 0) busy_start = now_ms();
+    if (idle && s->draining.empty() && busy_start >= 0) {
+      s->busy_ms += now_ms() - busy_start;
+      busy_start = -1;
+    }
+    if (s->stop && idle && s->draining.empty()) return;
+    if (!work) {
+      if (!s->draining.empty() || !s->queue.empty())
+        s->cv_work.wait_for(lk, std::chrono::microseconds(100));
+      else if (s->active.empty())
+        s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+    }
+  }
+}
+
+}  // namespace
+
 extern "C" {
-int d3h_server_create(d3h_context *, const d3h_server_config *, d3h_server **out) {
-  if (out) *out = nullptr;
-  return fail(D3H_ERR_UNSUPPORTED, "server not built yet");
+
+int d3h_server_create(d3h_context *ctx, const d3h_server_config *cfg_in, d3h_server **out) {
+  if (!ctx || !out) return fail(D3H_ERR_INVALID, "ctx/out is NULL");
+  *out = nullptr;
+  d3h_server_config cfg{};
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.arith != D3H_ARITH_EXACT && cfg.arith != D3H_ARITH_FMA) return fail(D3H_ERR_INVALID, "arith");
+  if (cfg.max_active <= 0) cfg.max_active = 4096;
+  if (cfg.temporal_depth <= 0) cfg.temporal_depth = 2;
+  if (cfg.temporal_depth != 1 && cfg.temporal_depth != 2 && cfg.temporal_depth != 4)
+    return fail(D3H_ERR_INVALID, "server depth %d (1, 2 or 4)", cfg.temporal_depth);
+  if (cfg.sweep_steps <= 0) cfg.sweep_steps = 8;
+  D3H_HIP(hipSetDevice(ctx->device));
+  size_t fr = 0, tot = 0;
+  D3H_HIP(hipMemGetInfo(&fr, &tot));
+  if (cfg.hbm_budget_bytes == 0) cfg.hbm_budget_bytes = fr / 10 * 9;
+  if (cfg.hbm_budget_bytes > fr) return fail(D3H_ERR_NOMEM, "budget %zu > free %zu", cfg.hbm_budget_bytes, fr);
+  auto *s = new d3h_server();
+  s->ctx = ctx;
+  s->cfg = cfg;
+  s->pool_bytes = align_up(cfg.hbm_budget_bytes);
+  bool ok = hipMalloc(&s->pool, s->pool_bytes) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) == hipSuccess;
+  for (int i = 0; ok && i < 2; ++i) {
+    hipStream_t st;
+    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+    if (ok) s->extra.push_back(st);
+  }
+  for (int i = 0; ok && i < 3; ++i) {
+    hipEvent_t ev;
+    ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    if (ok) s->fork_events.push_back(ev);
+  }
+  ok = ok && hipEventCreateWithFlags(&s->upload_ev, hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&s->sweep_ev[0], hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&s->sweep_ev[1], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    d3h_server_destroy(s);
+    return fail(D3H_ERR_NOMEM, "server resources (pool of %zu bytes)", s->pool_bytes);
+  }
+  s->alloc.reset(s->pool_bytes);
+  s->th = std::thread(scheduler, s);
+  *out = s;
+  return D3H_OK;
 }
-int d3h_server_destroy(d3h_server *) { return D3H_OK; }
-int d3h_server_submit(d3h_server *, const d3h_problem *, const void *, int, void *, int, int64_t *) {
-  return fail(D3H_ERR_UNSUPPORTED, "server");
+
+int d3h_server_destroy(d3h_server *s) {
+  if (!s) return D3H_OK;
+  if (s->th.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->th.join();
+  }
+  (void)hipSetDevice(s->ctx->device);
+  if (s->compute) (void)hipStreamSynchronize(s->compute);
+  if (s->copy) (void)hipStreamSynchronize(s->copy);
+  for (auto st : s->extra) (void)hipStreamDestroy(st);
+  for (auto ev : s->fork_events) (void)hipEventDestroy(ev);
+  if (s->upload_ev) (void)hipEventDestroy(s->upload_ev);
+  for (int i = 0; i < 2; ++i) {
+    if (s->sweep_ev[i]) (void)hipEventDestroy(s->sweep_ev[i]);
+    if (s->dev_tables[i]) (void)hipFree(s->dev_tables[i]);
+    if (s->host_tables[i]) (void)hipHostFree(s->host_tables[i]);
+  }
+  if (s->compute) (void)hipStreamDestroy(s->compute);
+  if (s->copy) (void)hipStreamDestroy(s->copy);
+  if (s->pool) (void)hipFree(s->pool);
+  for (auto &kv : s->reqs) delete kv.second;
+  delete s;
+  return D3H_OK;
 }
-int d3h_server_poll(d3h_server *, int64_t, int *, double *) { return fail(D3H_ERR_UNSUPPORTED, "server"); }
-int d3h_server_wait(d3h_server *, int64_t, double *) { return fail(D3H_ERR_UNSUPPORTED, "server"); }
-int d3h_server_stats(d3h_server *, int64_t *, int64_t *, int64_t *, double *) {
-  return fail(D3H_ERR_UNSUPPORTED, "server");
+
+int d3h_server_submit(d3h_server *s, const d3h_problem *p, const void *host_in, int host_dtype,
+                      void *host_out, int out_dtype, int64_t *id) {
+  if (!s || !p || !host_in || !host_out || !id) return fail(D3H_ERR_INVALID, "NULL argument");
+  if (!dtype_valid(host_dtype) || !dtype_valid(out_dtype)) return fail(D3H_ERR_INVALID, "dtype");
+  D3H_TRY(check_problem(*p));
+  auto *r = new Request();
+  r->p = *p;
+  r->in = host_in;
+  r->in_dt = host_dtype;
+  r->out = host_out;
+  r->out_dt = out_dtype;
+  r->pitch = pitch_elems(p->nx, p->dtype);
+  r->buf_bytes = align_up((size_t)r->pitch * p->ny * p->nz * dtype_bytes(p->dtype));
+  const size_t n = (size_t)p->nx * p->ny * p->nz;
+  size_t stage = 0;
+  if (host_dtype != p->dtype) stage = std::max(stage, n * dtype_bytes(host_dtype));
+  if (out_dtype != p->dtype) stage = std::max(stage, n * dtype_bytes(out_dtype));
+  r->footprint = 2 * r->buf_bytes + align_up(stage);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->fatal != D3H_OK) {
+    delete r;
+    return fail(s->fatal, "server stopped after an error");
+  }
+  if (r->footprint > s->pool_bytes) {
+    ++s->rejected;
+    delete r;
+    return fail(D3H_ERR_NOMEM, "request needs %zu bytes, server pool is %zu", r->footprint, s->pool_bytes);
+  }
+  r->id = s->next_id++;
+  r->t_submit = now_ms();
+  s->reqs[r->id] = r;
+  s->queue.push_back(r);
+  *id = r->id;
+  s->cv_work.notify_all();
+  return D3H_OK;
 }
+
+int d3h_server_poll(d3h_server *s, int64_t id, int *done, double *latency_ms) {
+  if (!s || !done) return fail(D3H_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(D3H_ERR_NOT_FOUND, "request %lld", (long long)id);
+  Request *r = it->second;
+  const int st = r->state.load();
+  *done = (st == kDone || st == kFailed) ? 1 : 0;
+  if (latency_ms) *latency_ms = st == kDone ? r->t_done - r->t_submit : 0.0;
+  if (st == kFailed) return fail(r->status, "request %lld failed", (long long)id);
+  return D3H_OK;
 }
+
+int d3h_server_wait(d3h_server *s, int64_t id, double *latency_ms) {
+  if (!s) return fail(D3H_ERR_INVALID, "NULL argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(D3H_ERR_NOT_FOUND, "request %lld", (long long)id);
+  Request *r = it->second;
+  s->cv_done.wait(lk, [&] {
+    const int st = r->state.load();
+    return st == kDone || st == kFailed;
+  });
+  if (latency_ms) *latency_ms = r->state.load() == kDone ? r->t_done - r->t_submit : 0.0;
+  if (r->state.load() == kFailed) return fail(r->status, "request %lld failed", (long long)id);
+  return D3H_OK;
+}
+
+int d3h_server_stats(d3h_server *s, int64_t *completed, int64_t *rejected, int64_t *sweeps,
+                     double *busy_ms) {
+  if (!s) return fail(D3H_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (completed) *completed = s->completed;
+  if (rejected) *rejected = s->rejected;
+  if (sweeps) *sweeps = s->sweeps;
+  if (busy_ms) *busy_ms = s->busy_ms;
+  return D3H_OK;
+}
+
+}  // extern "C"

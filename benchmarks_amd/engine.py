@@ -286,3 +286,62 @@ def stable_coefficients(nx, ny, nz, kappa=0.1, dtype=np.float32):
     s = f(f(f(f(f(ce + ce) + cn) + cn) + ct) + ct)
     cc = f(1.0 - float(s))
     return [float(v) for v in (ce, ce, cn, cn, ct, ct, cc)], float(dt)
+
+
+class Server:
+    """Continuous-batching request server (d3h_server_*)."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0, arith=N.ARITH_EXACT, depth=2,
+                 sweep_steps=8):
+        cfg = N.ServerConfig(hbm_budget_bytes, max_active, arith, depth, sweep_steps)
+        h = ctypes.c_void_p()
+        check(lib().d3h_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)),
+              "d3h_server_create")
+        self.ctx, self.handle = ctx, h
+        self._keep = {}
+
+    def close(self):
+        if self.handle:
+            lib().d3h_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, problem, host_in, host_dtype, out):
+        """Queue a solve; `out` (numpy, dense) receives the result.  Both arrays
+        are kept alive until the request is done."""
+        p = N.Problem()
+        p.nx, p.ny, p.nz = problem["nx"], problem["ny"], problem["nz"]
+        p.dtype, p.steps = problem.get("dtype", N.F32), problem["steps"]
+        for j, c in enumerate(problem["coef"]):
+            p.coef[j] = float(c)
+        out_dtype = {np.dtype(np.float32): N.F32, np.dtype(np.float64): N.F64,
+                     np.dtype(np.float16): N.F16, np.dtype(np.uint16): N.BF16}[out.dtype]
+        rid = ctypes.c_int64()
+        check(lib().d3h_server_submit(self.handle, ctypes.byref(p), _dptr(host_in), host_dtype,
+                                      _dptr(out), out_dtype, ctypes.byref(rid)), "d3h_server_submit")
+        self._keep[rid.value] = (host_in, out)
+        return rid.value
+
+    def poll(self, rid):
+        done, lat = ctypes.c_int(), ctypes.c_double()
+        check(lib().d3h_server_poll(self.handle, rid, ctypes.byref(done), ctypes.byref(lat)),
+              "d3h_server_poll")
+        return bool(done.value), lat.value
+
+    def wait(self, rid):
+        lat = ctypes.c_double()
+        check(lib().d3h_server_wait(self.handle, rid, ctypes.byref(lat)), "d3h_server_wait")
+        self._keep.pop(rid, None)
+        return lat.value
+
+    def stats(self):
+        c, r, s, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        check(lib().d3h_server_stats(self.handle, ctypes.byref(c), ctypes.byref(r), ctypes.byref(s),
+                                     ctypes.byref(b)), "d3h_server_stats")
+        return {"completed": c.value, "rejected": r.value, "sweeps": s.value, "busy_ms": b.value}
