@@ -21,6 +21,11 @@ cpu_baseline: the reference's OpenMP RunKernel (oracle/_ref/libref.so, built
 from the reference sources) or, when that library is absent, the oracle's C
 restatement, timed on this host's cores on a bounded sample of the same grid.
 
+Other workloads (--workload): plain512_f64, tb768_f32 (temporal blocking),
+batched512_f32 / batched512_bf16 (512 uneven grids per run, solves/s against
+launching the plain kernel per request) and served (request server: solves/s,
+p50/p99 latency under Poisson arrivals).  Each prints its own JSON line.
+
 Only one GPU is used: --gpus N > 1 exits with a message.
 """
 import argparse
@@ -29,6 +34,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -44,7 +51,6 @@ WORKLOADS = {
 
 def cpu_baseline(nx, ny, nz, steps_sample):
     """Time the reference's OpenMP RunKernel (or the oracle port) on the host."""
-    import numpy as np
     gomp = ctypes.CDLL("libgomp.so.1")
     gomp.omp_get_max_threads.restype = ctypes.c_int
     threads = gomp.omp_get_max_threads()
@@ -76,18 +82,183 @@ def cpu_baseline(nx, ny, nz, steps_sample):
                       % (nx, ny, nz, steps_sample, seconds)}
 
 
+def fp64_errors(d3, ctx, problems, inits, outs, sample):
+    """Relative L2 / max-abs of results against the FP64 GPU run of the same
+    problem (exact arithmetic), over a sample of problems."""
+    worst = {"rel_l2": 0.0, "max_abs": 0.0}
+    for i in sample:
+        p = problems[i]
+        with d3.Field(ctx, p["nx"], p["ny"], p["nz"], d3.F64) as ref, \
+                d3.Field(ctx, p["nx"], p["ny"], p["nz"], d3.F32) as got:
+            ref.upload(inits[i], d3.F32)
+            ref.run(p["steps"], p["coef"])
+            got.upload(outs[i], d3.F32)
+            c = got.compare(ref)
+        worst["rel_l2"] = max(worst["rel_l2"], c["rel_l2"])
+        worst["max_abs"] = max(worst["max_abs"], c["max_abs"])
+    return worst
+
+
+def bench_batched(args, d3, ctx, dname):
+    from benchmarks_amd import workloads
+    dtype = {"f32": d3.F32, "bf16": d3.BF16}[dname]
+    depth = args.depth if args.depth is not None else 2
+    arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
+    probs = workloads.batched_problems(dtype=dtype)
+    inits = [workloads.initial_field(p, i) for i, p in enumerate(probs)]
+    cells = sum(p["nx"] * p["ny"] * p["nz"] for p in probs)
+    steps = probs[0]["steps"]
+    # warm-up (kernels, allocator)
+    with d3.Batch(ctx, probs[:16]) as b:
+        for i in range(16):
+            b.upload(i, inits[i], d3.F32)
+        b.run(arith=arith, depth=depth, graph=True)
+    # end to end: create, upload, run (graph capture included), download
+    t0 = time.perf_counter()
+    b = d3.Batch(ctx, probs)
+    for i, f in enumerate(inits):
+        b.upload(i, f, d3.F32)
+    tm = b.run(arith=arith, depth=depth, graph=True)
+    outs = [b.download(i, d3.F32) for i in range(len(probs))]
+    t_e2e = time.perf_counter() - t0
+    # device-resident replay of the captured sequence
+    for i, f in enumerate(inits):
+        b.upload(i, f, d3.F32)
+    t1 = time.perf_counter()
+    tm2 = b.run(arith=arith, depth=depth, graph=True)
+    t_dev = time.perf_counter() - t1
+    b.close()
+    print("batched: e2e %.3f s, device %.3f s (%d launches)" % (t_e2e, t_dev, tm2["launches"]),
+          file=sys.stderr, flush=True)
+    # baseline: the plain kernel launched per request (alloc, H2D, launches, D2H)
+    t2 = time.perf_counter()
+    same = True
+    for i, p in enumerate(probs):
+        with d3.Field(ctx, p["nx"], p["ny"], p["nz"], dtype) as f:
+            f.upload(inits[i], d3.F32)
+            f.run(p["steps"], p["coef"], arith=arith)
+            o = f.download(d3.F32)
+        same = same and np.array_equal(o.view(np.uint32), outs[i].view(np.uint32))
+    t_base = time.perf_counter() - t2
+    err = fp64_errors(d3, ctx, probs, inits, outs, range(0, len(probs), 64))
+    n = len(probs)
+    return {
+        "metric": "batched solves/s (512 uneven grids, %d steps)" % steps,
+        "value": round(n / t_e2e, 2), "unit": "solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": round(t_e2e * 1e3 / steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dname,
+        "data": "synthetic (seeded dims log-uniform [32,256], cosine product + 1e-3 noise)",
+        "config": {"workload": "%d grids %s, %d steps, depth %d, %s" % (n, dname.upper(), steps, depth,
+                                                                          args.arith),
+                   "cells": cells, "parallelism": "single GPU"},
+        "device_resident_solves_per_s": round(n / t_dev, 2),
+        "device_resident_gcell_updates_per_s": round(cells * steps / t_dev / 1e9, 2),
+        "per_request_plain_solves_per_s": round(n / t_base, 2),
+        "speedup_vs_per_request": round(t_base / t_e2e, 2),
+        "results_equal_per_request": bool(same),
+        "error_vs_fp64": err,
+    }
+
+
+def bench_served(args, d3, ctx):
+    from benchmarks_amd import workloads
+    arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
+    depth = args.depth if args.depth is not None else 2
+    nreq = args.requests
+    arena = workloads.PinnedArena()
+    base = workloads.served_requests(nreq, lam=1.0)
+    ins, outs = [], []
+    for i, p in enumerate(base):
+        a = arena.empty((p["nz"], p["ny"], p["nx"]))
+        workloads.initial_field(p, i, out=a)
+        ins.append(a)
+        outs.append(arena.empty(a.shape))
+    work = sum(p["nx"] * p["ny"] * p["nz"] * p["steps"] for p in base)
+    srv = d3.Server(ctx, arith=arith, depth=depth, sweep_steps=args.sweep_steps)
+    # warm-up
+    w = srv.submit(base[0], ins[0], d3.F32, outs[0])
+    srv.wait(w)
+    # closed loop: everything submitted at once
+    t0 = time.perf_counter()
+    ids = [srv.submit(p, ins[i], d3.F32, outs[i]) for i, p in enumerate(base)]
+    for r in ids:
+        srv.wait(r)
+    t_closed = time.perf_counter() - t0
+    server_outs = [o.copy() for o in outs]
+    print("served closed loop: %d requests in %.3f s" % (nreq, t_closed), file=sys.stderr, flush=True)
+    # open loop at each arrival rate
+    rates = []
+    for lam in [float(x) for x in args.rates.split(",")]:
+        reqs = workloads.served_requests(nreq, lam=lam)
+        lat = []
+        t_start = time.perf_counter()
+        ids = []
+        for i, p in enumerate(reqs):
+            delay = p["arrival_s"] - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(p, ins[i], d3.F32, outs[i]))
+        for r in ids:
+            lat.append(srv.wait(r))
+        t_open = time.perf_counter() - t_start
+        lat = np.array(lat)
+        rates.append({"rate_per_s": lam, "solves_per_s": round(nreq / t_open, 2),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2)})
+        print("rate %.0f/s: %s" % (lam, rates[-1]), file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    # baseline: the plain kernel launched per request, one after another
+    t1 = time.perf_counter()
+    same = True
+    for i, p in enumerate(base):
+        with d3.Field(ctx, p["nx"], p["ny"], p["nz"], p["dtype"]) as f:
+            f.upload(ins[i], d3.F32)
+            f.run(p["steps"], p["coef"], arith=arith)
+            o = f.download(d3.F32)
+        same = same and np.array_equal(o.view(np.uint32), server_outs[i].view(np.uint32))
+    t_base = time.perf_counter() - t1
+    f32 = [i for i, p in enumerate(base) if p["dtype"] == d3.F32][:6]
+    bf = [i for i, p in enumerate(base) if p["dtype"] == d3.BF16][:6]
+    e32 = fp64_errors(d3, ctx, base, ins, server_outs, f32)
+    ebf = fp64_errors(d3, ctx, base, ins, server_outs, bf)
+    arena.close()
+    return {
+        "metric": "served solves/s, p50/p99 latency at equal error",
+        "value": round(nreq / t_closed, 2), "unit": "solves/s", "n_gpus": 1, "steps": nreq,
+        "warmup": 1, "ms_per_step": round(t_closed * 1e3 / nreq, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/bf16 storage, f32 compute",
+        "data": "synthetic (seeded requests: edge log-uniform [32,384], slab aspect, steps U[50,2000], "
+                "70% FP32 / 30% BF16)",
+        "config": {"workload": "%d requests, depth %d, %d steps per sweep, %s" % (
+                       nreq, depth, args.sweep_steps, args.arith),
+                   "cell_updates": work, "parallelism": "single GPU"},
+        "closed_loop_gcell_updates_per_s": round(work / t_closed / 1e9, 2),
+        "per_request_plain_solves_per_s": round(nreq / t_base, 2),
+        "speedup_vs_per_request": round(t_base / t_closed, 2),
+        "results_equal_per_request": bool(same),
+        "open_loop": rates,
+        "server_stats": st,
+        "error_vs_fp64": {"f32": e32, "bf16": ebf},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="plain512_f32", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="plain512_f32",
+                    choices=sorted(WORKLOADS) + ["batched512_f32", "batched512_bf16", "served"])
     ap.add_argument("--arith", default="exact", choices=["exact", "fma"])
     ap.add_argument("--depth", type=int, default=None, help="temporal depth (default per workload)")
     ap.add_argument("--graph", type=int, default=1, help="replay a captured hipGraph (1) or launch")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile before timing")
     ap.add_argument("--cpu-steps", type=int, default=500, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--requests", type=int, default=150, help="served: requests per run")
+    ap.add_argument("--rates", default="50,100,150", help="served: Poisson arrival rates (1/s)")
+    ap.add_argument("--sweep-steps", type=int, default=32, help="served: steps per scheduling round")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU (this build has no multi-GPU path); "
@@ -95,6 +266,15 @@ def main():
         return 2
 
     import benchmarks_amd as d3
+    if args.workload.startswith("batched") or args.workload == "served":
+        ctx = d3.Context(0)
+        if args.workload == "served":
+            line = bench_served(args, d3, ctx)
+        else:
+            line = bench_batched(args, d3, ctx, args.workload.split("_")[1])
+        print(json.dumps(line), flush=True)
+        ctx.close()
+        return 0
     nx, ny, nz, dname, depth, dsteps = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else dsteps
     depth = args.depth if args.depth is not None else depth

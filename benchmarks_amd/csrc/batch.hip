@@ -299,16 +299,9 @@ int d3h_batch_upload(d3h_batch *b, int i, const void *host, int host_dtype) {
   hipStream_t st = b->ctx->stream;
   D3H_HIP(hipSetDevice(b->ctx->device));
   char *dst = static_cast<char *>(b->pool) + b->off[i] + b->cur[i] * b->bytes[i];
-  const size_t n = (size_t)p.nx * p.ny * p.nz;
-  if (host_dtype == p.dtype) {
-    const size_t es = dtype_bytes(p.dtype);
-    D3H_HIP(hipMemcpy2DAsync(dst, b->pitch[i] * es, host, p.nx * es, p.nx * es, (size_t)p.ny * p.nz,
-                             hipMemcpyHostToDevice, st));
-  } else {
-    D3H_TRY(batch_staging(b, n * dtype_bytes(host_dtype)));
-    D3H_HIP(hipMemcpyAsync(b->staging, host, n * dtype_bytes(host_dtype), hipMemcpyHostToDevice, st));
-    D3H_TRY(launch_convert(b->staging, host_dtype, 0, dst, p.dtype, b->pitch[i], p.nx, p.ny, p.nz, st));
-  }
+  if (needs_staging(host_dtype, p.dtype, b->pitch[i], p.nx))
+    D3H_TRY(batch_staging(b, (size_t)p.nx * p.ny * p.nz * dtype_bytes(host_dtype)));
+  D3H_TRY(copy_in(host, host_dtype, dst, p.dtype, b->pitch[i], p.nx, p.ny, p.nz, b->staging, st));
   D3H_HIP(hipStreamSynchronize(st));
   return D3H_OK;
 }
@@ -320,16 +313,9 @@ int d3h_batch_download(d3h_batch *b, int i, void *host, int host_dtype) {
   hipStream_t st = b->ctx->stream;
   D3H_HIP(hipSetDevice(b->ctx->device));
   const char *src = static_cast<const char *>(b->pool) + b->off[i] + b->cur[i] * b->bytes[i];
-  const size_t n = (size_t)p.nx * p.ny * p.nz;
-  if (host_dtype == p.dtype) {
-    const size_t es = dtype_bytes(p.dtype);
-    D3H_HIP(hipMemcpy2DAsync(host, p.nx * es, src, b->pitch[i] * es, p.nx * es, (size_t)p.ny * p.nz,
-                             hipMemcpyDeviceToHost, st));
-  } else {
-    D3H_TRY(batch_staging(b, n * dtype_bytes(host_dtype)));
-    D3H_TRY(launch_convert(src, p.dtype, b->pitch[i], b->staging, host_dtype, 0, p.nx, p.ny, p.nz, st));
-    D3H_HIP(hipMemcpyAsync(host, b->staging, n * dtype_bytes(host_dtype), hipMemcpyDeviceToHost, st));
-  }
+  if (needs_staging(host_dtype, p.dtype, b->pitch[i], p.nx))
+    D3H_TRY(batch_staging(b, (size_t)p.nx * p.ny * p.nz * dtype_bytes(host_dtype)));
+  D3H_TRY(copy_out(src, p.dtype, b->pitch[i], host, host_dtype, p.nx, p.ny, p.nz, b->staging, st));
   D3H_HIP(hipStreamSynchronize(st));
   return D3H_OK;
 }

@@ -243,6 +243,35 @@ int launch_convert(const void *src, int src_dtype, int src_pitch, void *dst, int
   return D3H_OK;
 }
 
+bool needs_staging(int host_dtype, int dtype, int pitch, int nx) {
+  return host_dtype != dtype || pitch != nx;
+}
+
+int copy_in(const void *host, int host_dtype, void *dst, int dtype, int pitch, int nx, int ny, int nz,
+            void *staging, hipStream_t stream) {
+  const size_t n = (size_t)nx * ny * nz;
+  if (!needs_staging(host_dtype, dtype, pitch, nx)) {
+    D3H_HIP(hipMemcpyAsync(dst, host, n * dtype_bytes(dtype), hipMemcpyHostToDevice, stream));
+    return D3H_OK;
+  }
+  if (!staging) return fail(D3H_ERR_STATE, "copy_in needs a staging buffer");
+  D3H_HIP(hipMemcpyAsync(staging, host, n * dtype_bytes(host_dtype), hipMemcpyHostToDevice, stream));
+  return launch_convert(staging, host_dtype, 0, dst, dtype, pitch, nx, ny, nz, stream);
+}
+
+int copy_out(const void *src, int dtype, int pitch, void *host, int host_dtype, int nx, int ny, int nz,
+             void *staging, hipStream_t stream) {
+  const size_t n = (size_t)nx * ny * nz;
+  if (!needs_staging(host_dtype, dtype, pitch, nx)) {
+    D3H_HIP(hipMemcpyAsync(host, src, n * dtype_bytes(dtype), hipMemcpyDeviceToHost, stream));
+    return D3H_OK;
+  }
+  if (!staging) return fail(D3H_ERR_STATE, "copy_out needs a staging buffer");
+  D3H_TRY(launch_convert(src, dtype, pitch, staging, host_dtype, 0, nx, ny, nz, stream));
+  D3H_HIP(hipMemcpyAsync(host, staging, n * dtype_bytes(host_dtype), hipMemcpyDeviceToHost, stream));
+  return D3H_OK;
+}
+
 // ---- FP64 reductions --------------------------------------------------------
 constexpr int kRedThreads = 256;
 
@@ -521,17 +550,10 @@ int d3h_field_upload(d3h_field *f, const void *host, int host_dtype) {
   d3h_context *ctx = f->ctx;
   D3H_HIP(hipSetDevice(ctx->device));
   const size_t n = (size_t)f->nx * f->ny * f->nz;
-  if (host_dtype == f->dtype) {
-    const size_t es = dtype_bytes(f->dtype);
-    D3H_HIP(hipMemcpy2DAsync(f->buf[f->cur], f->pitch * es, host, f->nx * es, f->nx * es,
-                             (size_t)f->ny * f->nz, hipMemcpyHostToDevice, ctx->stream));
-  } else {
+  if (needs_staging(host_dtype, f->dtype, f->pitch, f->nx))
     D3H_TRY(ensure_staging(f, n * dtype_bytes(host_dtype)));
-    D3H_HIP(hipMemcpyAsync(f->staging, host, n * dtype_bytes(host_dtype), hipMemcpyHostToDevice,
-                           ctx->stream));
-    D3H_TRY(launch_convert(f->staging, host_dtype, 0, f->buf[f->cur], f->dtype, f->pitch, f->nx, f->ny,
-                           f->nz, ctx->stream));
-  }
+  D3H_TRY(copy_in(host, host_dtype, f->buf[f->cur], f->dtype, f->pitch, f->nx, f->ny, f->nz, f->staging,
+                  ctx->stream));
   D3H_HIP(hipStreamSynchronize(ctx->stream));
   return D3H_OK;
 }
@@ -542,17 +564,10 @@ int d3h_field_download(d3h_field *f, void *host, int host_dtype) {
   d3h_context *ctx = f->ctx;
   D3H_HIP(hipSetDevice(ctx->device));
   const size_t n = (size_t)f->nx * f->ny * f->nz;
-  if (host_dtype == f->dtype) {
-    const size_t es = dtype_bytes(f->dtype);
-    D3H_HIP(hipMemcpy2DAsync(host, f->nx * es, f->buf[f->cur], f->pitch * es, f->nx * es,
-                             (size_t)f->ny * f->nz, hipMemcpyDeviceToHost, ctx->stream));
-  } else {
+  if (needs_staging(host_dtype, f->dtype, f->pitch, f->nx))
     D3H_TRY(ensure_staging(f, n * dtype_bytes(host_dtype)));
-    D3H_TRY(launch_convert(f->buf[f->cur], f->dtype, f->pitch, f->staging, host_dtype, 0, f->nx, f->ny,
-                           f->nz, ctx->stream));
-    D3H_HIP(hipMemcpyAsync(host, f->staging, n * dtype_bytes(host_dtype), hipMemcpyDeviceToHost,
-                           ctx->stream));
-  }
+  D3H_TRY(copy_out(f->buf[f->cur], f->dtype, f->pitch, host, host_dtype, f->nx, f->ny, f->nz, f->staging,
+                   ctx->stream));
   D3H_HIP(hipStreamSynchronize(ctx->stream));
   return D3H_OK;
 }

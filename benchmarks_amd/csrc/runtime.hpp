@@ -60,6 +60,15 @@ struct TuneVal {
 // Size class of a grid: ceil(log2) of each dimension.
 TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz);
 
+// Host <-> pitched device field.  One dense 1-D copy when the dtypes match and
+// rows are unpadded; otherwise a dense 1-D copy through `staging` (at least
+// nx*ny*nz elements of the host dtype) and an on-device conversion/repitch.
+// Strided 2-D copies from pageable memory are avoided: they run row by row.
+bool needs_staging(int host_dtype, int dtype, int pitch, int nx);
+int copy_in(const void *host, int host_dtype, void *dst, int dtype, int pitch, int nx, int ny, int nz,
+            void *staging, hipStream_t stream);
+int copy_out(const void *src, int dtype, int pitch, void *host, int host_dtype, int nx, int ny, int nz,
+             void *staging, hipStream_t stream);
 }  // namespace d3h
 
 struct d3h_context {

@@ -174,34 +174,17 @@ char *field_ptr(d3h_server *s, Request *r, int which) {
 
 int upload(d3h_server *s, Request *r) {
   const d3h_problem &p = r->p;
-  const size_t n = (size_t)p.nx * p.ny * p.nz;
-  char *dst = field_ptr(s, r, 0);
-  if (r->in_dt == p.dtype) {
-    const size_t es = dtype_bytes(p.dtype);
-    D3H_HIP(hipMemcpy2DAsync(dst, r->pitch * es, r->in, p.nx * es, p.nx * es, (size_t)p.ny * p.nz,
-                             hipMemcpyHostToDevice, s->copy));
-  } else {
-    char *stage = static_cast<char *>(s->pool) + r->stage_off;
-    D3H_HIP(hipMemcpyAsync(stage, r->in, n * dtype_bytes(r->in_dt), hipMemcpyHostToDevice, s->copy));
-    D3H_TRY(launch_convert(stage, r->in_dt, 0, dst, p.dtype, r->pitch, p.nx, p.ny, p.nz, s->copy));
-  }
+  char *stage = static_cast<char *>(s->pool) + r->stage_off;
+  D3H_TRY(copy_in(r->in, r->in_dt, field_ptr(s, r, 0), p.dtype, r->pitch, p.nx, p.ny, p.nz, stage, s->copy));
   r->cur = 0;
   return D3H_OK;
 }
 
 int download(d3h_server *s, Request *r) {
   const d3h_problem &p = r->p;
-  const size_t n = (size_t)p.nx * p.ny * p.nz;
-  const char *src = field_ptr(s, r, r->cur);
-  if (r->out_dt == p.dtype) {
-    const size_t es = dtype_bytes(p.dtype);
-    D3H_HIP(hipMemcpy2DAsync(r->out, p.nx * es, src, r->pitch * es, p.nx * es, (size_t)p.ny * p.nz,
-                             hipMemcpyDeviceToHost, s->copy));
-  } else {
-    char *stage = static_cast<char *>(s->pool) + r->stage_off;
-    D3H_TRY(launch_convert(src, p.dtype, r->pitch, stage, r->out_dt, 0, p.nx, p.ny, p.nz, s->copy));
-    D3H_HIP(hipMemcpyAsync(r->out, stage, n * dtype_bytes(r->out_dt), hipMemcpyDeviceToHost, s->copy));
-  }
+  char *stage = static_cast<char *>(s->pool) + r->stage_off;
+  D3H_TRY(copy_out(field_ptr(s, r, r->cur), p.dtype, r->pitch, r->out, r->out_dt, p.nx, p.ny, p.nz, stage,
+                   s->copy));
   auto *arg = new std::pair<d3h_server *, Request *>(s, r);
   hipError_t e = hipLaunchHostFunc(s->copy, on_request_done, arg);
   if (e != hipSuccess) {
@@ -434,8 +417,8 @@ int d3h_server_submit(d3h_server *s, const d3h_problem *p, const void *host_in, 
   r->buf_bytes = align_up((size_t)r->pitch * p->ny * p->nz * dtype_bytes(p->dtype));
   const size_t n = (size_t)p->nx * p->ny * p->nz;
   size_t stage = 0;
-  if (host_dtype != p->dtype) stage = std::max(stage, n * dtype_bytes(host_dtype));
-  if (out_dtype != p->dtype) stage = std::max(stage, n * dtype_bytes(out_dtype));
+  if (needs_staging(host_dtype, p->dtype, r->pitch, p->nx)) stage = std::max(stage, n * dtype_bytes(host_dtype));
+  if (needs_staging(out_dtype, p->dtype, r->pitch, p->nx)) stage = std::max(stage, n * dtype_bytes(out_dtype));
   r->footprint = 2 * r->buf_bytes + align_up(stage);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->fatal != D3H_OK) {
