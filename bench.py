@@ -330,11 +330,20 @@ def main():
     # for fused launches the algorithmic bytes are those of the single step
     # sequence they replace (effective bandwidth)
     achieved = bytes_per_launch * steps_per_launch / avg_launch_s / 1e9
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
+    # against the copy probe) when the profile is of the kernel that ran here
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            prof = json.load(fh)
+        t = d3.tiles()[tm["tile"]]
+        vec = t["tile_x"] // (64 * t["groups"])
+        name = "stencil_kernel<%d, %d, %d, %d, %d, %d, %s>" % (
+            t["dtype"], vec, t["temporal_depth"], t["groups"], t["rows_per_wave"], t["waves"],
+            "true" if t["arith"] else "false")
+        if prof.get("kernel") == name:
+            traffic = prof.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
