@@ -339,9 +339,9 @@ def main():
             prof = json.load(fh)
         t = d3.tiles()[tm["tile"]]
         vec = t["tile_x"] // (64 * t["groups"])
-        name = "stencil_kernel<%d, %d, %d, %d, %d, %d, %s>" % (
+        name = "stencil_kernel<%d, %d, %d, %d, %d, %d, %s, %d>" % (
             t["dtype"], vec, t["temporal_depth"], t["groups"], t["rows_per_wave"], t["waves"],
-            "true" if t["arith"] else "false")
+            "true" if t["arith"] else "false", t["prefetch"])
         if prof.get("kernel") == name:
             traffic = prof.get("hbm_bytes_per_launch")
 

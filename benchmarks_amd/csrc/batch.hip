@@ -37,7 +37,8 @@ const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, in
     if (it != ctx->tuned.end()) {
       for (const auto &e : registry())
         if (e.launch_batch && e.dtype == dtype && e.T == T && e.fma == fma && e.vec == it->second.vec &&
-            e.G == it->second.G && e.RPW == it->second.RPW && e.NW == it->second.NW)
+            e.G == it->second.G && e.RPW == it->second.RPW && e.NW == it->second.NW &&
+            e.pf == it->second.pf)
           return &e;
     }
   }

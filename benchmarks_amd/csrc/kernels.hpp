@@ -21,6 +21,7 @@ struct KernelEntry {
   int RPW;    // rows per wave
   int NW;     // waves per workgroup
   int fma;    // 0 = exact, 1 = fma chain
+  int pf;     // prefetch distance in planes
   const void *func;
   LaunchFn launch;
   const void *batch_func;      // nullptr when no batched instantiation exists
@@ -29,36 +30,37 @@ struct KernelEntry {
   int tile_y() const { return NW * RPW; }
 };
 
-template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF>
 void launch_stencil(const void *src, void *dst, const Geometry &g, const double *c, unsigned blocks,
                     hipStream_t stream) {
   using C = typename Traits<S>::C;
   using ST = typename Traits<S>::T;
   Coef<C> k{(C)c[0], (C)c[1], (C)c[2], (C)c[3], (C)c[4], (C)c[5], (C)c[6]};
-  hipLaunchKernelGGL((stencil_kernel<S, VEC, T, G, RPW, NW, FMA>), dim3(blocks), dim3(64 * NW), 0,
+  hipLaunchKernelGGL((stencil_kernel<S, VEC, T, G, RPW, NW, FMA, PF>), dim3(blocks), dim3(64 * NW), 0,
                      stream, (const ST *)src, (ST *)dst, g, k);
 }
 
-template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF>
 void launch_stencil_batch(const BatchMember *members, const int *block_member, int launch,
                           unsigned blocks, hipStream_t stream) {
-  hipLaunchKernelGGL((stencil_batch_kernel<S, VEC, T, G, RPW, NW, FMA>), dim3(blocks), dim3(64 * NW),
+  hipLaunchKernelGGL((stencil_batch_kernel<S, VEC, T, G, RPW, NW, FMA, PF>), dim3(blocks), dim3(64 * NW),
                      0, stream, members, block_member, launch);
 }
 
-// Single-grid kernel only.
-#define D3H_ENTRY(S, VEC, T, G, RPW, NW, FMA)                                               \
-  KernelEntry {                                                                             \
-    S, VEC, T, G, RPW, NW, FMA, (const void *)&stencil_kernel<S, VEC, T, G, RPW, NW, FMA>,  \
-        &launch_stencil<S, VEC, T, G, RPW, NW, FMA>, nullptr, nullptr                       \
+// Single-grid kernel only (prefetch distance PF).
+#define D3H_ENTRY_PF(S, VEC, T, G, RPW, NW, FMA, PF)                                          \
+  KernelEntry {                                                                               \
+    S, VEC, T, G, RPW, NW, FMA, PF, (const void *)&stencil_kernel<S, VEC, T, G, RPW, NW, FMA, PF>, \
+        &launch_stencil<S, VEC, T, G, RPW, NW, FMA, PF>, nullptr, nullptr                     \
   }
+#define D3H_ENTRY(S, VEC, T, G, RPW, NW, FMA) D3H_ENTRY_PF(S, VEC, T, G, RPW, NW, FMA, 1)
 // Single-grid and batched kernels.
-#define D3H_ENTRY_B(S, VEC, T, G, RPW, NW, FMA)                                             \
-  KernelEntry {                                                                             \
-    S, VEC, T, G, RPW, NW, FMA, (const void *)&stencil_kernel<S, VEC, T, G, RPW, NW, FMA>,  \
-        &launch_stencil<S, VEC, T, G, RPW, NW, FMA>,                                        \
-        (const void *)&stencil_batch_kernel<S, VEC, T, G, RPW, NW, FMA>,                    \
-        &launch_stencil_batch<S, VEC, T, G, RPW, NW, FMA>                                   \
+#define D3H_ENTRY_B(S, VEC, T, G, RPW, NW, FMA)                                               \
+  KernelEntry {                                                                               \
+    S, VEC, T, G, RPW, NW, FMA, 1, (const void *)&stencil_kernel<S, VEC, T, G, RPW, NW, FMA, 1>, \
+        &launch_stencil<S, VEC, T, G, RPW, NW, FMA, 1>,                                       \
+        (const void *)&stencil_batch_kernel<S, VEC, T, G, RPW, NW, FMA, 1>,                   \
+        &launch_stencil_batch<S, VEC, T, G, RPW, NW, FMA, 1>                                  \
   }
 
 // Each kernels_*.hip file defines one table.

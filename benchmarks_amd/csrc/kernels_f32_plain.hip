@@ -15,7 +15,12 @@ namespace d3h {
   D3H_ENTRY(kF32, 4, 1, 1, 4, 1, A), D3H_ENTRY_B(kF32, 2, 1, 1, 4, 2, A),           \
   D3H_ENTRY_B(kF32, 2, 1, 1, 2, 4, A), D3H_ENTRY(kF32, 4, 1, 3, 2, 4, A),          \
   D3H_ENTRY(kF32, 4, 1, 3, 1, 8, A), D3H_ENTRY(kF32, 4, 1, 3, 2, 2, A),            \
-  D3H_ENTRY(kF32, 4, 1, 3, 1, 4, A)
+  D3H_ENTRY(kF32, 4, 1, 3, 1, 4, A),                                               \
+  /* deeper prefetch (2-3 planes in flight per wave) */                          \
+  D3H_ENTRY_PF(kF32, 4, 1, 2, 1, 4, A, 2), D3H_ENTRY_PF(kF32, 4, 1, 2, 1, 4, A, 3), \
+  D3H_ENTRY_PF(kF32, 4, 1, 2, 2, 4, A, 2), D3H_ENTRY_PF(kF32, 4, 1, 2, 1, 8, A, 2), \
+  D3H_ENTRY_PF(kF32, 4, 1, 2, 2, 2, A, 2), D3H_ENTRY_PF(kF32, 4, 1, 3, 1, 4, A, 2), \
+  D3H_ENTRY_PF(kF32, 4, 1, 2, 4, 1, A, 2)
 
 static const KernelEntry kTable[] = {F32_PLAIN(false), F32_PLAIN(true)};
 

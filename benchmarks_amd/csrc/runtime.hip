@@ -129,10 +129,10 @@ TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz) {
   return TuneKey{dtype, T, fma, ilog2ceil(nx), ilog2ceil(ny), ilog2ceil(nz)};
 }
 
-static const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW) {
+static const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf) {
   for (const auto &e : registry())
     if (e.dtype == dtype && e.T == T && e.fma == fma && e.vec == vec && e.G == G && e.RPW == RPW &&
-        e.NW == NW)
+        e.NW == NW && e.pf == pf)
       return &e;
   return nullptr;
 }
@@ -159,7 +159,7 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
     auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz));
     if (it != ctx->tuned.end()) {
       const KernelEntry *e = find_entry(dtype, T, fma, it->second.vec, it->second.G, it->second.RPW,
-                                        it->second.NW);
+                                        it->second.NW, it->second.pf);
       if (e) {
         *entry = e;
         *zchunks = cfg.zchunks > 0 ? cfg.zchunks : it->second.zchunks;
@@ -426,15 +426,15 @@ int d3h_context_create(int device, d3h_context **out) {
 
 int d3h_context_destroy(d3h_context *c) {
   if (!c) return D3H_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->copy) hipStreamSynchronize(c->copy);
-  if (c->partials) hipFree(c->partials);
-  if (c->host_partials) hipHostFree(c->host_partials);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
-  if (c->copy) hipStreamDestroy(c->copy);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->copy) (void)hipStreamSynchronize(c->copy);
+  if (c->partials) (void)hipFree(c->partials);
+  if (c->host_partials) (void)hipHostFree(c->host_partials);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->copy) (void)hipStreamDestroy(c->copy);
   delete c;
   return D3H_OK;
 }
@@ -466,7 +466,7 @@ int d3h_host_free(void *ptr) {
 }
 
 static void release_graph(d3h_field *f) {
-  if (f->gexec) hipGraphExecDestroy(f->gexec);
+  if (f->gexec) (void)hipGraphExecDestroy(f->gexec);
   f->gexec = nullptr;
   f->gkey.clear();
 }
@@ -501,12 +501,12 @@ int d3h_field_create(d3h_context *ctx, int nx, int ny, int nz, int dtype, d3h_fi
 
 int d3h_field_destroy(d3h_field *f) {
   if (!f) return D3H_OK;
-  hipSetDevice(f->ctx->device);
-  hipStreamSynchronize(f->ctx->stream);
+  (void)hipSetDevice(f->ctx->device);
+  (void)hipStreamSynchronize(f->ctx->stream);
   release_graph(f);
   for (int i = 0; i < 2; ++i)
-    if (f->buf[i]) hipFree(f->buf[i]);
-  if (f->staging) hipFree(f->staging);
+    if (f->buf[i]) (void)hipFree(f->buf[i]);
+  if (f->staging) (void)hipFree(f->staging);
   delete f;
   return D3H_OK;
 }
@@ -522,7 +522,7 @@ int d3h_field_dims(const d3h_field *f, int *nx, int *ny, int *nz, int *dtype) {
 
 static int ensure_staging(d3h_field *f, size_t bytes) {
   if (f->staging_bytes >= bytes) return D3H_OK;
-  if (f->staging) hipFree(f->staging);
+  if (f->staging) (void)hipFree(f->staging);
   f->staging = nullptr;
   f->staging_bytes = 0;
   if (hipMalloc(&f->staging, bytes) != hipSuccess)
@@ -593,12 +593,12 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
         hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
         f->cur = cur0;
         if (rc != D3H_OK) {
-          if (graph) hipGraphDestroy(graph);
+          if (graph) (void)hipGraphDestroy(graph);
           return rc;
         }
         if (ce != hipSuccess) return fail(D3H_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
         hipError_t ie = hipGraphInstantiate(&f->gexec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (ie != hipSuccess) return fail(D3H_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
         key.push_back(launches);
         key.push_back(blocks);
@@ -681,7 +681,7 @@ int d3h_field_error_analytical(const d3h_field *f, double kappa, double time, co
   D3H_HIP(hipMalloc(&dtab, sizeof(double) * tab.size()));
   hipError_t e = hipMemcpy(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    hipFree(dtab);
+    (void)hipFree(dtab);
     return fail(D3H_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
   }
   const int nb = ctx->partial_blocks;
@@ -692,7 +692,7 @@ int d3h_field_error_analytical(const d3h_field *f, double kappa, double time, co
     e = hipMemcpyAsync(ctx->host_partials, ctx->partials, sizeof(double) * 2 * nb,
                        hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  hipFree(dtab);
+  (void)hipFree(dtab);
   if (e != hipSuccess) return fail(D3H_ERR_HIP, "analytical norm: %s", hipGetErrorString(e));
   double d2 = 0, mx = 0;
   for (int i = 0; i < nb; ++i) {
@@ -719,6 +719,7 @@ int d3h_tile_info(int index, d3h_tile_desc *out) {
   out->waves = e.NW;
   out->tile_x = e.tile_x();
   out->tile_y = e.tile_y();
+  out->prefetch = e.pf;
   return D3H_OK;
 }
 
@@ -756,18 +757,18 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
       if (make_geometry(e, nx, ny, nz, f->pitch, zc, ctx->num_cus, occ, &g2, &nb2) != D3H_OK) continue;
       e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);  // warm-up
       f->cur ^= 1;
-      hipEventRecord(ctx->ev0, ctx->stream);
+      (void)hipEventRecord(ctx->ev0, ctx->stream);
       for (int r = 0; r < reps; ++r) {
         e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);
         f->cur ^= 1;
       }
-      hipEventRecord(ctx->ev1, ctx->stream);
+      (void)hipEventRecord(ctx->ev1, ctx->stream);
       if (hipEventSynchronize(ctx->ev1) != hipSuccess || hipGetLastError() != hipSuccess) {
         rc = fail(D3H_ERR_HIP, "autotune launch of tile %d failed", i);
         break;
       }
       float ms = 0;
-      hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+      (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
       const double per_step = ms / reps / T;
       if (per_step < best) {
         best = per_step;
@@ -782,7 +783,7 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     const KernelEntry &e = reg[bi];
-    ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] = TuneVal{e.vec, e.G, e.RPW, e.NW, bz, best};
+    ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] = TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz, best};
   }
   if (best_tile) *best_tile = bi;
   if (best_zchunks) *best_zchunks = bz;
@@ -794,13 +795,14 @@ int d3h_autotune_save(d3h_context *ctx, const char *path) {
   if (!ctx || !path) return fail(D3H_ERR_INVALID, "NULL argument");
   std::ofstream os(path);
   if (!os) return fail(D3H_ERR_INVALID, "cannot write %s", path);
-  os << "# dtype depth fma log2nx log2ny log2nz vec groups rows_per_wave waves zchunks ms_per_step\n";
+  os << "# dtype depth fma log2nx log2ny log2nz vec groups rows_per_wave waves prefetch zchunks "
+        "ms_per_step\n";
   std::lock_guard<std::mutex> lk(ctx->mu);
   for (const auto &kv : ctx->tuned) {
     const auto &k = kv.first;
     const auto &v = kv.second;
     os << k.dtype << ' ' << k.T << ' ' << k.fma << ' ' << k.cx << ' ' << k.cy << ' ' << k.cz << ' '
-       << v.vec << ' ' << v.G << ' ' << v.RPW << ' ' << v.NW << ' ' << v.zchunks << ' '
+       << v.vec << ' ' << v.G << ' ' << v.RPW << ' ' << v.NW << ' ' << v.pf << ' ' << v.zchunks << ' '
        << v.ms_per_step << '\n';
   }
   return os ? D3H_OK : fail(D3H_ERR_INVALID, "write to %s failed", path);
@@ -818,9 +820,9 @@ int d3h_autotune_load(d3h_context *ctx, const char *path) {
     TuneKey k{};
     TuneVal v{};
     if (!(ss >> k.dtype >> k.T >> k.fma >> k.cx >> k.cy >> k.cz >> v.vec >> v.G >> v.RPW >> v.NW >>
-          v.zchunks >> v.ms_per_step))
+          v.pf >> v.zchunks >> v.ms_per_step))
       return fail(D3H_ERR_INVALID, "malformed line in %s: %s", path, line.c_str());
-    if (find_entry(k.dtype, k.T, k.fma, v.vec, v.G, v.RPW, v.NW)) ctx->tuned[k] = v;
+    if (find_entry(k.dtype, k.T, k.fma, v.vec, v.G, v.RPW, v.NW, v.pf)) ctx->tuned[k] = v;
   }
   return D3H_OK;
 }
@@ -833,31 +835,31 @@ int d3h_bandwidth_probe(d3h_context *ctx, size_t n, int dtype, int reps, double 
   void *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess) return fail(D3H_ERR_NOMEM, "probe alloc");
   if (hipMalloc(&b, bytes) != hipSuccess) {
-    hipFree(a);
+    (void)hipFree(a);
     return fail(D3H_ERR_NOMEM, "probe alloc");
   }
-  hipMemsetAsync(a, 0, bytes, ctx->stream);
+  (void)hipMemsetAsync(a, 0, bytes, ctx->stream);
   const long long nvec = (long long)(bytes / 16);
   if ((nvec + 255) / 256 > INT_MAX) {
-    hipFree(a);
-    hipFree(b);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return fail(D3H_ERR_INVALID, "probe size too large");
   }
   const unsigned blocks = (unsigned)((nvec + 255) / 256);
   hipLaunchKernelGGL(scale_copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (const u32x4 *)a,
                      (u32x4 *)b, nvec, 0.4f);
-  hipEventRecord(ctx->ev0, ctx->stream);
+  (void)hipEventRecord(ctx->ev0, ctx->stream);
   for (int r = 0; r < reps; ++r) {
     hipLaunchKernelGGL(scale_copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
                        (const u32x4 *)((r & 1) ? b : a), (u32x4 *)((r & 1) ? a : b), nvec, 0.4f);
   }
-  hipEventRecord(ctx->ev1, ctx->stream);
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
   hipError_t e = hipEventSynchronize(ctx->ev1);
   if (e == hipSuccess) e = hipGetLastError();
   float ms = 0;
-  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  hipFree(a);
-  hipFree(b);
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   if (e != hipSuccess) return fail(D3H_ERR_HIP, "probe: %s", hipGetErrorString(e));
   const double avg = ms / reps;
   if (avg_ms) *avg_ms = avg;

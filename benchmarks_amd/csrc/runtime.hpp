@@ -53,7 +53,7 @@ struct TuneKey {
   }
 };
 struct TuneVal {
-  int vec, G, RPW, NW;  // entry identity (stable across builds)
+  int vec, G, RPW, NW, pf;  // entry identity (stable across builds)
   int zchunks;
   double ms_per_step;
 };

@@ -222,7 +222,8 @@ __device__ __forceinline__ C update(const Coef<C> &k, C c, C w, C e, C s, C n, C
   }
 }
 
-constexpr int mod4(int v) { return ((v % 4) + 4) % 4; }
+template <int N>
+constexpr int modn(int v) { return ((v % N) + N) % N; }
 
 // Exchange buffer between the waves of a workgroup: [buf][level][wave][side][v][lane].
 // Double-buffered (one barrier per plane) when both buffers fit in 64 KB,
@@ -237,7 +238,10 @@ struct XchLayout {
   }
 };
 
-template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+// PF = prefetch distance in planes: plane i + PF is requested while plane i
+// is computed.  Each level keeps NS = PF + 3 register slots (plane q in slot
+// q mod NS) and the z loop is unrolled by NS.
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF>
 struct TileRunner {
   using ST = typename Traits<S>::T;
   using C = typename Traits<S>::C;
@@ -247,9 +251,10 @@ struct TileRunner {
   static constexpr int GSTRIDE = 64 * VEC;  // cells between a lane's groups
   static constexpr int TILE_X = G * GSTRIDE;
   static constexpr int TILE_Y = NW * RPW;
+  static constexpr int NS = PF + 3;
 
   // lv[level][slot][row][cell]: level 0 is the loaded input.
-  C lv[T][4][RPW][NV];
+  C lv[T][NS][RPW][NV];
 
   const ST *__restrict__ src;
   ST *__restrict__ dst;
@@ -301,7 +306,7 @@ struct TileRunner {
 #pragma unroll
     for (int l = 0; l < T; ++l)
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
 #pragma unroll
@@ -336,12 +341,11 @@ struct TileRunner {
   // Publish the first/last row of every level's centre plane for iteration
   // phase PH (plane i - s of level s - 1 for stage s).
   template <int PH>
-  __device__ __forceinline__ void publish() {
+  __device__ __forceinline__ void publish(int buf) {
     if constexpr (NW > 1) {
-      constexpr int buf = (X::kBufs == 2) ? (PH & 1) : 0;
 #pragma unroll
       for (int s = 1; s <= T; ++s) {
-        const int sc = mod4(PH - s);
+        const int sc = modn<NS>(PH - s);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           xch[X::idx(buf, s - 1, wave, 0, v, lane)] = lv[s - 1][sc][0][v];
@@ -352,9 +356,8 @@ struct TileRunner {
   }
 
   template <int PH, int s, bool EDGE>
-  __device__ __forceinline__ void stage_impl(int p) {
-    constexpr int sc = mod4(PH - s), sb = mod4(PH - s - 1), st = mod4(PH - s + 1);
-    constexpr int buf = (X::kBufs == 2) ? (PH & 1) : 0;
+  __device__ __forceinline__ void stage_impl(int p, int buf) {
+    constexpr int sc = modn<NS>(PH - s), sb = modn<NS>(PH - s - 1), st = modn<NS>(PH - s + 1);
     const bool zb = (p == 0), zt = (p == geo.nz - 1);
     const int wn = max(wave - 1, 0), ws = min(wave + 1, NW - 1);
     C out[RPW][NV];
@@ -437,38 +440,45 @@ struct TileRunner {
   // values that are never stored or read by a valid plane), so each path
   // through an iteration issues the same memory instructions.
   template <int PH, int s>
-  __device__ __forceinline__ void stage(int i) {
+  __device__ __forceinline__ void stage(int i, int buf) {
     const int p = i - s;
-    if (edge_xy || p <= 0 || p >= geo.nz - 1) stage_impl<PH, s, true>(p);
-    else stage_impl<PH, s, false>(p);
+    if (edge_xy || p <= 0 || p >= geo.nz - 1) stage_impl<PH, s, true>(p, buf);
+    else stage_impl<PH, s, false>(p, buf);
   }
 
   template <int PH, int s>
-  __device__ __forceinline__ void stages(int i) {
-    stage<PH, s>(i);
-    if constexpr (s < T) stages<PH, s + 1>(i);
+  __device__ __forceinline__ void stages(int i, int buf) {
+    stage<PH, s>(i, buf);
+    if constexpr (s < T) stages<PH, s + 1>(i, buf);
+  }
+
+  // Iteration i (i == PH mod NS): request plane i + PF, exchange the wave
+  // edge rows, run the stages.  The exchange buffer alternates with i.
+  template <int PH>
+  __device__ __forceinline__ void iter(int i) {
+    load_plane<modn<NS>(PH + PF)>(i + PF);
+    const int buf = (X::kBufs == 2) ? (i & 1) : 0;
+    if constexpr (NW > 1) {
+      if constexpr (X::kBufs == 1) __syncthreads();  // previous plane's reads are done
+      publish<PH>(buf);
+      __syncthreads();
+    }
+    stages<PH, 1>(i, buf);
   }
 
   template <int PH>
-  __device__ __forceinline__ void iter(int i) {
-    load_plane<mod4(PH + 1)>(i + 1);
-    if constexpr (NW > 1) {
-      if constexpr (X::kBufs == 1) __syncthreads();  // previous plane's reads are done
-      publish<PH>();
-      __syncthreads();
-    }
-    stages<PH, 1>(i);
+  __device__ __forceinline__ void iters(int i) {
+    iter<PH>(i + PH);
+    if constexpr (PH + 1 < NS) iters<PH + 1>(i);
   }
 
   __device__ __forceinline__ void run() {
-    const int i0 = (zl0 - 1) & ~3;  // floor to a multiple of 4 (also for -1)
+    // the first plane (zl0) is requested at iteration zl0 - PF; start at the
+    // multiple of NS at or below it so phases are compile-time
+    const int a = zl0 - PF;
+    const int i0 = a - modn<NS>(a);
     const int ilast = zo1 - 1 + T;
-    for (int i = i0; i <= ilast; i += 4) {
-      iter<0>(i);
-      iter<1>(i + 1);
-      iter<2>(i + 2);
-      iter<3>(i + 3);
-    }
+    for (int i = i0; i <= ilast; i += NS) iters<0>(i);
   }
 };
 
@@ -480,11 +490,11 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
   return xcd * q + min(xcd, r) + slot;
 }
 
-template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF = 1>
 __global__ void __launch_bounds__(64 * NW)
 stencil_kernel(const typename Traits<S>::T *__restrict__ src, typename Traits<S>::T *__restrict__ dst,
                Geometry geo, Coef<typename Traits<S>::C> k) {
-  using R = TileRunner<S, VEC, T, G, RPW, NW, FMA>;
+  using R = TileRunner<S, VEC, T, G, RPW, NW, FMA, PF>;
   __shared__ typename Traits<S>::C xch[R::X::kElems];
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   const int tx = lb % geo.ntx;
@@ -509,11 +519,11 @@ struct BatchMember {
 // Batched launch: logical block -> member through a block table (one scalar
 // load), then the same tile runner.  Launch `launch` of a sequence reads
 // buf[(parity0 + launch) & 1] and writes the other buffer.
-template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA>
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF = 1>
 __global__ void __launch_bounds__(64 * NW)
 stencil_batch_kernel(const BatchMember *__restrict__ members, const int *__restrict__ block_member,
                      int launch) {
-  using R = TileRunner<S, VEC, T, G, RPW, NW, FMA>;
+  using R = TileRunner<S, VEC, T, G, RPW, NW, FMA, PF>;
   using C = typename Traits<S>::C;
   using ST = typename Traits<S>::T;
   __shared__ C xch[R::X::kElems];

@@ -135,6 +135,7 @@ typedef struct {
   int rows_per_wave;   /* rows each wavefront owns                          */
   int waves;           /* wavefronts per workgroup                          */
   int tile_x, tile_y;  /* loaded tile extent in cells / rows               */
+  int prefetch;        /* planes requested ahead of the one being computed  */
 } d3h_tile_desc;
 int d3h_tile_count(void);
 int d3h_tile_info(int index, d3h_tile_desc *out);
