@@ -80,6 +80,7 @@ SIGNATURES = {
     "d3h_field_dims": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int)]),
     "d3h_field_upload": (c_int, [c_void_p, c_void_p, c_int]),
     "d3h_field_download": (c_int, [c_void_p, c_void_p, c_int]),
+    "d3h_field_set_y_reversed": (c_int, [c_void_p, c_int]),
     "d3h_field_run": (c_int, [c_void_p, c_int, P(c_double), P(RunConfig), P(Timing)]),
     "d3h_field_compare": (c_int, [c_void_p, c_void_p, P(c_double)]),
     "d3h_field_error_analytical": (c_int, [c_void_p, c_double, c_double, P(c_double), P(c_double),
@@ -95,6 +96,8 @@ SIGNATURES = {
     "d3h_constants_f64": (c_int, [c_int, c_int, c_int, P(c_double)]),
     "d3h_initialize_f32": (c_int, [P(c_float), c_int, c_int, c_int, P(c_float), c_float]),
     "d3h_initialize_f64": (c_int, [P(c_double), c_int, c_int, c_int, P(c_double), c_double]),
+    "d3h_constants2d_f32": (c_int, [c_int, c_int, P(c_float)]),
+    "d3h_initialize2d_f32": (c_int, [P(c_float), c_int, c_int, P(c_float), c_float]),
     "d3h_add_noise_f32": (c_int, [P(c_float), c_size_t, c_uint64, c_float]),
     "d3h_batch_create": (c_int, [c_void_p, P(Problem), c_int, P(c_void_p)]),
     "d3h_batch_destroy": (c_int, [c_void_p]),

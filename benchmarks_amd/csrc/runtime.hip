@@ -222,52 +222,60 @@ __device__ __forceinline__ void store_elem(void *p, int dtype, size_t i, double 
   }
 }
 
+// Copies rows of nx cells; with ny > 0 the destination's rows of every plane
+// are in reverse order (row y of plane z goes to row ny-1-y).
 __global__ void convert_kernel(const void *src, int sdt, int sp, void *dst, int ddt, int dp, int nx,
-                               long long rows) {
+                               long long rows, int ny_rev) {
   const long long n = rows * nx;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const long long row = i / nx, x = i - row * nx;
-    store_elem(dst, ddt, (size_t)(row * dp + x), load_elem(src, sdt, (size_t)(row * sp + x)));
+    long long drow = row;
+    if (ny_rev > 0) {
+      const long long z = row / ny_rev, y = row - z * ny_rev;
+      drow = z * ny_rev + (ny_rev - 1 - y);
+    }
+    store_elem(dst, ddt, (size_t)(drow * dp + x), load_elem(src, sdt, (size_t)(row * sp + x)));
   }
 }
 
 int launch_convert(const void *src, int src_dtype, int src_pitch, void *dst, int dst_dtype,
-                   int dst_pitch, int nx, int ny, int nz, hipStream_t stream) {
+                   int dst_pitch, int nx, int ny, int nz, hipStream_t stream, bool yrev) {
   const long long rows = (long long)ny * nz;
   const long long n = rows * nx;
   const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(convert_kernel, dim3(blocks), dim3(256), 0, stream, src, src_dtype,
-                     src_pitch ? src_pitch : nx, dst, dst_dtype, dst_pitch ? dst_pitch : nx, nx, rows);
+                     src_pitch ? src_pitch : nx, dst, dst_dtype, dst_pitch ? dst_pitch : nx, nx, rows,
+                     yrev ? ny : 0);
   D3H_HIP(hipGetLastError());
   return D3H_OK;
 }
 
-bool needs_staging(int host_dtype, int dtype, int pitch, int nx) {
-  return host_dtype != dtype || pitch != nx;
+bool needs_staging(int host_dtype, int dtype, int pitch, int nx, bool yrev) {
+  return host_dtype != dtype || pitch != nx || yrev;
 }
 
 int copy_in(const void *host, int host_dtype, void *dst, int dtype, int pitch, int nx, int ny, int nz,
-            void *staging, hipStream_t stream) {
+            void *staging, hipStream_t stream, bool yrev) {
   const size_t n = (size_t)nx * ny * nz;
-  if (!needs_staging(host_dtype, dtype, pitch, nx)) {
+  if (!needs_staging(host_dtype, dtype, pitch, nx, yrev)) {
     D3H_HIP(hipMemcpyAsync(dst, host, n * dtype_bytes(dtype), hipMemcpyHostToDevice, stream));
     return D3H_OK;
   }
   if (!staging) return fail(D3H_ERR_STATE, "copy_in needs a staging buffer");
   D3H_HIP(hipMemcpyAsync(staging, host, n * dtype_bytes(host_dtype), hipMemcpyHostToDevice, stream));
-  return launch_convert(staging, host_dtype, 0, dst, dtype, pitch, nx, ny, nz, stream);
+  return launch_convert(staging, host_dtype, 0, dst, dtype, pitch, nx, ny, nz, stream, yrev);
 }
 
 int copy_out(const void *src, int dtype, int pitch, void *host, int host_dtype, int nx, int ny, int nz,
-             void *staging, hipStream_t stream) {
+             void *staging, hipStream_t stream, bool yrev) {
   const size_t n = (size_t)nx * ny * nz;
-  if (!needs_staging(host_dtype, dtype, pitch, nx)) {
+  if (!needs_staging(host_dtype, dtype, pitch, nx, yrev)) {
     D3H_HIP(hipMemcpyAsync(host, src, n * dtype_bytes(dtype), hipMemcpyDeviceToHost, stream));
     return D3H_OK;
   }
   if (!staging) return fail(D3H_ERR_STATE, "copy_out needs a staging buffer");
-  D3H_TRY(launch_convert(src, dtype, pitch, staging, host_dtype, 0, nx, ny, nz, stream));
+  D3H_TRY(launch_convert(src, dtype, pitch, staging, host_dtype, 0, nx, ny, nz, stream, yrev));
   D3H_HIP(hipMemcpyAsync(host, staging, n * dtype_bytes(host_dtype), hipMemcpyDeviceToHost, stream));
   return D3H_OK;
 }
@@ -511,6 +519,12 @@ int d3h_field_destroy(d3h_field *f) {
   return D3H_OK;
 }
 
+int d3h_field_set_y_reversed(d3h_field *f, int reversed) {
+  if (!f) return fail(D3H_ERR_INVALID, "field is NULL");
+  f->yrev = reversed != 0;
+  return D3H_OK;
+}
+
 int d3h_field_dims(const d3h_field *f, int *nx, int *ny, int *nz, int *dtype) {
   if (!f) return fail(D3H_ERR_INVALID, "field is NULL");
   if (nx) *nx = f->nx;
@@ -537,10 +551,10 @@ int d3h_field_upload(d3h_field *f, const void *host, int host_dtype) {
   d3h_context *ctx = f->ctx;
   D3H_HIP(hipSetDevice(ctx->device));
   const size_t n = (size_t)f->nx * f->ny * f->nz;
-  if (needs_staging(host_dtype, f->dtype, f->pitch, f->nx))
+  if (needs_staging(host_dtype, f->dtype, f->pitch, f->nx, f->yrev))
     D3H_TRY(ensure_staging(f, n * dtype_bytes(host_dtype)));
   D3H_TRY(copy_in(host, host_dtype, f->buf[f->cur], f->dtype, f->pitch, f->nx, f->ny, f->nz, f->staging,
-                  ctx->stream));
+                  ctx->stream, f->yrev));
   D3H_HIP(hipStreamSynchronize(ctx->stream));
   return D3H_OK;
 }
@@ -551,10 +565,10 @@ int d3h_field_download(d3h_field *f, void *host, int host_dtype) {
   d3h_context *ctx = f->ctx;
   D3H_HIP(hipSetDevice(ctx->device));
   const size_t n = (size_t)f->nx * f->ny * f->nz;
-  if (needs_staging(host_dtype, f->dtype, f->pitch, f->nx))
+  if (needs_staging(host_dtype, f->dtype, f->pitch, f->nx, f->yrev))
     D3H_TRY(ensure_staging(f, n * dtype_bytes(host_dtype)));
   D3H_TRY(copy_out(f->buf[f->cur], f->dtype, f->pitch, host, host_dtype, f->nx, f->ny, f->nz, f->staging,
-                   ctx->stream));
+                   ctx->stream, f->yrev));
   D3H_HIP(hipStreamSynchronize(ctx->stream));
   return D3H_OK;
 }

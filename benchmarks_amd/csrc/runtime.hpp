@@ -64,11 +64,13 @@ TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz);
 // rows are unpadded; otherwise a dense 1-D copy through `staging` (at least
 // nx*ny*nz elements of the host dtype) and an on-device conversion/repitch.
 // Strided 2-D copies from pageable memory are avoided: they run row by row.
-bool needs_staging(int host_dtype, int dtype, int pitch, int nx);
+// yrev: the device field holds the rows of every plane in reverse order (the
+// host's row y is device row ny-1-y).
+bool needs_staging(int host_dtype, int dtype, int pitch, int nx, bool yrev = false);
 int copy_in(const void *host, int host_dtype, void *dst, int dtype, int pitch, int nx, int ny, int nz,
-            void *staging, hipStream_t stream);
+            void *staging, hipStream_t stream, bool yrev = false);
 int copy_out(const void *src, int dtype, int pitch, void *host, int host_dtype, int nx, int ny, int nz,
-             void *staging, hipStream_t stream);
+             void *staging, hipStream_t stream, bool yrev = false);
 }  // namespace d3h
 
 struct d3h_context {
@@ -93,6 +95,7 @@ struct d3h_field {
   int cur = 0;
   void *staging = nullptr;
   size_t staging_bytes = 0;
+  bool yrev = false;  // rows stored in reverse order (see d3h_field_set_y_reversed)
   // cached captured launch sequence
   hipGraphExec_t gexec = nullptr;
   std::vector<long long> gkey;
@@ -109,5 +112,5 @@ int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEnt
 // 0 means dense (pitch = nx).  Narrowing rounds to nearest even (from f64 to
 // bf16/f16 it rounds to f32 first).
 int launch_convert(const void *src, int src_dtype, int src_pitch, void *dst, int dst_dtype,
-                   int dst_pitch, int nx, int ny, int nz, hipStream_t stream);
+                   int dst_pitch, int nx, int ny, int nz, hipStream_t stream, bool yrev = false);
 }  // namespace d3h

@@ -69,6 +69,26 @@ def initialize_f64(nx, ny, nz, constants, time=0.0, out=None):
     return out
 
 
+def constants2d_f32(nx, ny):
+    """Constants of the diffusion/ family's 2D benchmark, Diffusion(2, {nx, ny})
+    (diffusion/diffusion.h:93-108), in the same 15-slot order; ct = cb = 0."""
+    out = np.zeros(15, dtype=np.float32)
+    check(lib().d3h_constants2d_f32(nx, ny, out.ctypes.data_as(N.P(ctypes.c_float))),
+          "d3h_constants2d_f32")
+    return out
+
+
+def initialize2d_f32(nx, ny, constants, time=0.0, out=None):
+    """2D cosine product (diffusion/diffusion.h:48-77, ndim = 2) as (ny, nx)."""
+    if out is None:
+        out = np.empty((ny, nx), dtype=np.float32)
+    k = np.ascontiguousarray(constants, dtype=np.float32)
+    check(lib().d3h_initialize2d_f32(out.ctypes.data_as(N.P(ctypes.c_float)), nx, ny,
+                                     k.ctypes.data_as(N.P(ctypes.c_float)), float(time)),
+          "d3h_initialize2d_f32")
+    return out
+
+
 def add_noise_f32(buf, seed, amp):
     check(lib().d3h_add_noise_f32(buf.ctypes.data_as(N.P(ctypes.c_float)), buf.size, seed,
                                   float(amp)), "d3h_add_noise_f32")
@@ -168,6 +188,12 @@ class Field:
                                                                          DTYPE_NAMES[host_dtype]))
         check(lib().d3h_field_upload(self.handle, _dptr(host), host_dtype), "d3h_field_upload")
 
+    def set_y_reversed(self, reversed_=True):
+        """Keep the rows of every plane reversed on the device so the y-1 term
+        is summed before the y+1 term (the diffusion/ family's order)."""
+        check(lib().d3h_field_set_y_reversed(self.handle, 1 if reversed_ else 0),
+              "d3h_field_set_y_reversed")
+
     def download(self, host_dtype=None):
         if host_dtype is None:
             host_dtype = self.dtype
@@ -194,6 +220,21 @@ class Field:
         check(lib().d3h_field_error_analytical(self.handle, kappa, time, kk, dd, out),
               "d3h_field_error_analytical")
         return {"rms": out[0], "max_abs": out[1]}
+
+
+def run_kernel2d(count, f1, coef):
+    """The diffusion/ family's 2D RunKernel(count) (diffusion/diffusion_openmp.cc:59-82)
+    on a float32 (ny, nx) array, in place: an nx x ny x 1 device field with
+    reversed rows, advanced by the 7-point kernel with ct = cb = 0."""
+    ny, nx = f1.shape
+    c = [float(v) for v in coef]
+    c[4] = c[5] = 0.0
+    with Context() as ctx, Field(ctx, nx, ny, 1) as f:
+        f.set_y_reversed(True)
+        f.upload(f1)
+        f.run(count, c)
+        f1[...] = f.download().reshape(ny, nx)
+    return f1
 
 
 def run_kernel(count, f1, coef):

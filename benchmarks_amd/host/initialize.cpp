@@ -104,6 +104,50 @@ int d3h_initialize_f64(double *buf, int nx, int ny, int nz, const double k[15], 
   return D3H_OK;
 }
 
+// 2D benchmark of the diffusion/ family: Diffusion(2, {nx, ny})
+// (diffusion/diffusion.h:93-108): nz = 1, dz = 1; the sum in cc includes
+// `(ndim == 3 ? (ct + cb) : 0)`, i.e. a float 0.  The 2D operator has no t/b
+// terms, so ct = cb = 0 here (the reference computes them but never uses them).
+int d3h_constants2d_f32(int nx, int ny, float o[15]) {
+  if (nx <= 0 || ny <= 0 || !o) return D3H_ERR_INVALID;
+  const float kappa = (float)0.1;
+  const float l = 1.0f;
+  const float dx = l / (float)nx, dy = l / (float)ny, dz = l / 1.0f;
+  const float kx = (float)(2.0 * M_PI);
+  const float dt = (float)(0.1 * (double)dx * (double)dx / (double)kappa);
+  const float ce = kappa * dt / (dx * dx);
+  const float cn = kappa * dt / (dy * dy);
+  float sum = ce + ce;
+  sum = sum + cn;
+  sum = sum + cn;
+  sum = sum + 0.0f;
+  const float cc = (float)(1.0 - (double)sum);
+  const float v[15] = {kappa, dx, dy, dz, kx, kx, kx, dt, ce, ce, cn, cn, 0.0f, 0.0f, cc};
+  for (int i = 0; i < 15; ++i) o[i] = v[i];
+  return D3H_OK;
+}
+
+// 2D cosine product (diffusion/diffusion.h:48-77 with ndim = 2):
+// (REAL)0.125 * (1.0 - ax*cos(kx*x)) * (1.0 - ay*cos(ky*y)), double products.
+int d3h_initialize2d_f32(float *buf, int nx, int ny, const float k[15], float time) {
+  if (!buf || !k || nx <= 0 || ny <= 0) return D3H_ERR_INVALID;
+  const float kappa = k[0], dx = k[1], dy = k[2];
+  const float kx = k[4], ky = k[5];
+  const float ax = expf(-kappa * time * (kx * kx));
+  const float ay = expf(-kappa * time * (ky * ky));
+#pragma omp parallel for schedule(static)
+  for (int jy = 0; jy < ny; jy++) {
+    const float y = dy * (float)((double)jy + 0.5);
+    const double fy = 1.0 - (double)(ay * cosf(ky * y));
+    for (int jx = 0; jx < nx; jx++) {
+      const float x = dx * (float)((double)jx + 0.5);
+      double f0 = (double)0.125f * (1.0 - (double)(ax * cosf(kx * x)));
+      buf[(size_t)jy * nx + jx] = (float)(f0 * fy);
+    }
+  }
+  return D3H_OK;
+}
+
 // Counter-based noise: SplitMix64 finaliser of seed * golden + index + 1; the
 // top 24 bits give u in [0, 1) exactly; noise = amp * (2u - 1) added in float.
 static inline uint64_t mix64(uint64_t z) {

@@ -114,6 +114,13 @@ int d3h_field_dims(const d3h_field *f, int *nx, int *ny, int *nz, int *dtype);
  * (converted on the device, round-to-nearest-even when narrowing). */
 int d3h_field_upload(d3h_field *f, const void *host, int host_dtype);
 int d3h_field_download(d3h_field *f, void *host, int host_dtype);
+/* Store the rows of every plane in reverse order on the device (uploads and
+ * downloads flip them).  The kernels sum the y+1 term before the y-1 term as
+ * diffusion3d does (diffusion3d/baseline.cc:39-40); the diffusion/ family
+ * names y the other way round and sums y-1 first (diffusion/baseline.h:59-67),
+ * so its results are reproduced bitwise on a reversed field.  The stencil is
+ * symmetric in y, so only the rounding order changes. */
+int d3h_field_set_y_reversed(d3h_field *f, int reversed);
 /* Advance the field by exactly `count` steps on the device (synchronous). */
 int d3h_field_run(d3h_field *f, int count, const double coef[7],
                   const d3h_run_config *cfg, d3h_timing *timing);
@@ -168,6 +175,13 @@ int d3h_initialize_f32(float *buf, int nx, int ny, int nz,
                        const float constants[15], float time);
 int d3h_initialize_f64(double *buf, int nx, int ny, int nz,
                        const double constants[15], double time);
+/* 2D form of the diffusion/ family (nz = 1, no t/b terms): constants of
+ * Diffusion(2, {nx, ny}) (diffusion/diffusion.h:93-108; ct = cb = 0 and
+ * cc = 1 - (ce + cw + cn + cs)) and the 2D cosine product
+ * (diffusion/diffusion.h:48-77 with ndim = 2).  Advance such a field with
+ * d3h_field_run on an nx x ny x 1 field (y reversed) and these coefficients. */
+int d3h_constants2d_f32(int nx, int ny, float constants[15]);
+int d3h_initialize2d_f32(float *buf, int nx, int ny, const float constants[15], float time);
 /* Cosine product plus counter-based noise U(-amp, amp) keyed by (seed, index). */
 int d3h_add_noise_f32(float *buf, size_t n, uint64_t seed, float amp);
 

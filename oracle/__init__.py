@@ -50,10 +50,14 @@ def lib():
             "oracle_rms_vs_analytical_f64": [_d, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d,
                                              ctypes.c_int],
             "oracle_add_noise_f32": [_f, ctypes.c_long, ctypes.c_uint64, ctypes.c_float],
+            "oracle_constants2d_f32": [ctypes.c_int, ctypes.c_int, _f],
+            "oracle_initialize2d_f32": [_f, ctypes.c_int, ctypes.c_int, _f, ctypes.c_float],
+            "oracle_run2d_f32": [_f, _f, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int],
+            "oracle_accuracy2d_f32": [_f, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int],
             "oracle_diff_norms_f32": [_f, _f, ctypes.c_long, _d],
             "oracle_diff_norms_f64": [_d, _d, ctypes.c_long, _d],
         }
-        res = {"oracle_accuracy_f32": ctypes.c_float, "oracle_rms_vs_analytical_f32": ctypes.c_double,
+        res = {"oracle_accuracy_f32": ctypes.c_float, "oracle_accuracy2d_f32": ctypes.c_float, "oracle_rms_vs_analytical_f32": ctypes.c_double,
                "oracle_rms_vs_analytical_f64": ctypes.c_double}
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -164,6 +168,37 @@ def rms_vs_analytical_f64(f, k, count):
     f = np.ascontiguousarray(f, np.float64)
     k = np.ascontiguousarray(k, np.float64)
     return float(lib().oracle_rms_vs_analytical_f64(_p(f, _d), nx, ny, nz, _p(k, _d), count))
+
+
+def constants2d_f32(nx, ny):
+    o = np.zeros(15, np.float32)
+    lib().oracle_constants2d_f32(nx, ny, _p(o, _f))
+    return o
+
+
+def initialize2d_f32(nx, ny, k, time=0.0):
+    out = np.empty((ny, nx), np.float32)
+    k = np.ascontiguousarray(k, np.float32)
+    lib().oracle_initialize2d_f32(_p(out, _f), nx, ny, _p(k, _f), float(time))
+    return out
+
+
+def run2d_f32(f, coef, count):
+    """`count` 5-point steps of a float32 (ny, nx) field (returns a new array);
+    coef in the 7-slot order, ct and cb ignored."""
+    a = np.array(f, dtype=np.float32, copy=True, order="C")
+    b = np.empty_like(a)
+    c = np.ascontiguousarray(coef, np.float32)
+    ny, nx = a.shape
+    lib().oracle_run2d_f32(_p(a, _f), _p(b, _f), nx, ny, _p(c, _f), count)
+    return a
+
+
+def accuracy2d_f32(f, k, count):
+    ny, nx = f.shape
+    f = np.ascontiguousarray(f, np.float32)
+    k = np.ascontiguousarray(k, np.float32)
+    return float(lib().oracle_accuracy2d_f32(_p(f, _f), nx, ny, _p(k, _f), count))
 
 
 def add_noise_f32(f, seed, amp):

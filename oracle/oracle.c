@@ -305,6 +305,80 @@ double oracle_rms_vs_analytical_f64(const double *f, int nx, int ny, int nz,
 }
 
 /* ---------------------------------------------------------------------------
+ * 2D operator of the diffusion/ family
+ * ------------------------------------------------------------------------- */
+void oracle_constants2d_f32(int nx, int ny, float o[ORACLE_NCONST]) {
+  /* diffusion/diffusion.h:93-108 with ndim = 2 (nz = 1) */
+  float kappa = (float)0.1;
+  float l = 1.0f;
+  float dx = l / (float)nx, dy = l / (float)ny, dz = l / 1.0f;
+  float kx = (float)(2.0 * M_PI);
+  float dt = (float)(0.1 * (double)dx * (double)dx / (double)kappa);
+  float ce = kappa * dt / (dx * dx);
+  float cn = kappa * dt / (dy * dy);
+  /* 1.0 - (ce + cw + cn + cs + (ndim == 3 ? (ct + cb) : 0)) */
+  float s = ce + ce;
+  s = s + cn;
+  s = s + cn;
+  s = s + 0.0f;
+  float cc = (float)(1.0 - (double)s);
+  o[0] = kappa; o[1] = dx; o[2] = dy; o[3] = dz;
+  o[4] = kx; o[5] = kx; o[6] = kx; o[7] = dt;
+  o[8] = ce; o[9] = ce; o[10] = cn; o[11] = cn; o[12] = 0.0f; o[13] = 0.0f;
+  o[14] = cc;
+}
+
+void oracle_initialize2d_f32(float *buf, int nx, int ny, const float k[ORACLE_NCONST], float time) {
+  /* diffusion/diffusion.h:48-77, ndim = 2 */
+  const float kappa = k[0], dx = k[1], dy = k[2], kx = k[4], ky = k[5];
+  const float ax = expf(-kappa * time * (kx * kx));
+  const float ay = expf(-kappa * time * (ky * ky));
+  for (int jy = 0; jy < ny; jy++) {
+    for (int jx = 0; jx < nx; jx++) {
+      float x = dx * (float)((double)jx + 0.5);
+      float y = dy * (float)((double)jy + 0.5);
+      double f0 = (double)0.125f * (1.0 - (double)(ax * cosf(kx * x)));
+      f0 = f0 * (1.0 - (double)(ay * cosf(ky * y)));
+      buf[(long)jy * nx + jx] = (float)f0;
+    }
+  }
+}
+
+void oracle_run2d_f32(float *f1, float *f2, int nx, int ny, const float c[ORACLE_NCOEF], int count) {
+  /* diffusion/diffusion_openmp.cc:59-82: s = y-1, n = y+1 */
+  const float ce = c[0], cw = c[1], cn = c[2], cs = c[3], cc = c[6];
+  float *a = f1, *b = f2;
+  for (int it = 0; it < count; ++it) {
+#pragma omp parallel for
+    for (int y = 0; y < ny; y++) {
+      for (int x = 0; x < nx; x++) {
+        long ci = x + (long)y * nx;
+        long w = (x == 0) ? ci : ci - 1;
+        long e = (x == nx - 1) ? ci : ci + 1;
+        long s = (y == 0) ? ci : ci - nx;
+        long n = (y == ny - 1) ? ci : ci + nx;
+        b[ci] = cc * a[ci] + cw * a[w] + ce * a[e] + cs * a[s] + cn * a[n];
+      }
+    }
+    float *t = a; a = b; b = t;
+  }
+  if (a != f1) memcpy(f1, a, sizeof(float) * (size_t)nx * ny);
+}
+
+float oracle_accuracy2d_f32(const float *f, int nx, int ny, const float k[ORACLE_NCONST], int count) {
+  const long len = (long)nx * ny;
+  float *ref = (float *)malloc(sizeof(float) * (size_t)len);
+  oracle_initialize2d_f32(ref, nx, ny, k, (float)count * k[7]);
+  float err = 0.0f;
+  for (long i = 0; i < len; i++) {
+    float diff = ref[i] - f[i];
+    err += diff * diff;
+  }
+  free(ref);
+  return sqrtf(err / (float)len);
+}
+
+/* ---------------------------------------------------------------------------
  * Seeded noise (counter-based SplitMix64 finaliser keyed by seed and index).
  * ------------------------------------------------------------------------- */
 static inline uint64_t mix64(uint64_t z) {

@@ -73,6 +73,17 @@ double oracle_rms_vs_analytical_f64(const double *f, int nx, int ny, int nz,
  * counterpart (the reference initialises with the cosine product only). */
 void oracle_add_noise_f32(float *buf, long n, uint64_t seed, float amp);
 
+/* 2D 5-point operator of the diffusion/ family (nz = 1, no t/b terms):
+ * constants of Diffusion(2, {nx, ny}) (diffusion/diffusion.h:93-108, ct = cb
+ * reported as 0), the 2D cosine product (diffusion/diffusion.h:48-77), the
+ * step with that family's y naming, s = y-1 summed before n = y+1
+ * (diffusion/diffusion_openmp.cc:59-82), and its accuracy (diffusion/baseline.h
+ * GetAccuracy, float accumulator). */
+void oracle_constants2d_f32(int nx, int ny, float out[ORACLE_NCONST]);
+void oracle_initialize2d_f32(float *buf, int nx, int ny, const float k[ORACLE_NCONST], float time);
+void oracle_run2d_f32(float *f1, float *f2, int nx, int ny, const float c[ORACLE_NCOEF], int count);
+float oracle_accuracy2d_f32(const float *f, int nx, int ny, const float k[ORACLE_NCONST], int count);
+
 /* Relative L2 and max-abs difference in FP64 (metric helpers). */
 void oracle_diff_norms_f32(const float *a, const float *b, long n, double out[3]);
 void oracle_diff_norms_f64(const double *a, const double *b, long n, double out[3]);
