@@ -1,8 +1,9 @@
 // diffusion3d_hip: the reference's benchmark driver (diffusion3d/main.cc:84-184)
 // with the HIP engine as the variant.  Same options (--count N, --size N,
 // --dump, --help; unknown options are ignored like the reference's
-// getopt loop), plus --warmup (diffusion/main.cc:102-109) and --nx/--ny/--nz
-// for non-cubic grids.  Engine knobs come from the environment:
+// getopt loop), plus --warmup and --nd N (diffusion/main.cc:86-129; --nd 2
+// runs the diffusion/ family's 2D benchmark) and --nx/--ny/--nz for
+// non-cubic grids.  Engine knobs come from the environment:
 // D3H_TEMPORAL_DEPTH (1, 2, 4, 8), D3H_ARITH (0 exact, 1 fma),
 // D3H_USE_GRAPH (0/1), D3H_DEVICE.
 #include <getopt.h>
@@ -10,6 +11,7 @@
 
 #include <iostream>
 
+#include "diffusion2d.hpp"
 #include "diffusion3d.hpp"
 
 using namespace diffusion3d;
@@ -17,6 +19,7 @@ using namespace diffusion3d;
 static void PrintUsage(std::ostream &os, const char *prog) {
   os << "Usage: " << prog << " [options] [benchmarks]\n\n";
   os << "Options\n"
+     << "\t--nd N      Number of dimensions, 2 or 3 (default: 3)\n"
      << "\t--count N   Number of iterations\n"
      << "\t--size N    Size of each dimension\n"
      << "\t--nx N      Size of x (overrides --size)\n"
@@ -28,12 +31,12 @@ static void PrintUsage(std::ostream &os, const char *prog) {
 }
 
 int main(int argc, char *argv[]) {
-  int size = 256, count = 100, nx = 0, ny = 0, nz = 0;
+  int size = 256, count = 100, nx = 0, ny = 0, nz = 0, nd = 3;
   bool dump = false, warmup = false;
   static struct option long_options[] = {
       {"count", 1, 0, 0}, {"size", 1, 0, 0}, {"dump", 0, 0, 0}, {"help", 0, 0, 0},
       {"warmup", 0, 0, 0}, {"nx", 1, 0, 0}, {"ny", 1, 0, 0},   {"nz", 1, 0, 0},
-      {0, 0, 0, 0}};
+      {"nd", 1, 0, 0},     {0, 0, 0, 0}};
   while (1) {
     int option_index = 0;
     int c = getopt_long(argc, argv, "", long_options, &option_index);
@@ -48,15 +51,21 @@ int main(int argc, char *argv[]) {
       case 5: nx = atoi(optarg); break;
       case 6: ny = atoi(optarg); break;
       case 7: nz = atoi(optarg); break;
+      case 8: nd = atoi(optarg); break;
       default: break;
     }
   }
   if (nx <= 0) nx = size;
   if (ny <= 0) ny = size;
   if (nz <= 0) nz = size;
-  if (nx <= 0 || ny <= 0 || nz <= 0 || count < 0) {
-    std::cerr << "invalid size or count\n";
+  if (nx <= 0 || ny <= 0 || nz <= 0 || count < 0 || (nd != 2 && nd != 3)) {
+    std::cerr << "invalid size, count or number of dimensions\n";
     return EXIT_FAILURE;
+  }
+  if (nd == 2) {
+    diffusion::Diffusion2DHIP bmk(nx, ny);
+    bmk.RunBenchmark(count, dump, warmup);
+    return 0;
   }
   Diffusion3D *bmk = new Diffusion3DHIP(nx, ny, nz, argc, argv);
   bmk->RunBenchmark(count, dump, warmup);

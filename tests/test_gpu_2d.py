@@ -127,3 +127,26 @@ def test_family_3d_fixtures_bitwise(ctx, golden2d):
             f.run(case["count"], d3.coefficients(c))
             out = f.download()
         check_case(out, case, arrays, "out3d_")
+
+
+def test_driver_nd2_matches_family_output(golden2d, tmp_path):
+    """`diffusion3d_hip --nd 2` prints the diffusion/ family's lines, its
+    accuracy for 256x256 / 100 steps, and dumps the family's "%.7E" file."""
+    import subprocess
+    from benchmarks_amd import _native as N
+    meta, _ = golden2d
+    case = [c for c in meta["cases2d"] if c["shape"] == [256, 256] and c["count"] == 100][0]
+    r = subprocess.run([N.DRIVER_PATH, "--nd", "2", "--size", "256", "--count", "100", "--dump",
+                        "--warmup"], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    for line in ("*** Diffusion Benchmark ***", "Grid size: 256x256", "Running the kernel...",
+                 "Kernel-only performance:"):
+        assert line in out, out
+    acc = [l for l in out.splitlines() if l.startswith("Accuracy")][0].split(":")[1].strip()
+    assert acc == case["accuracy"], out
+    lines = (tmp_path / "diffusion_result.hip.out").read_text().split()
+    assert len(lines) == 256 * 256
+    samples = np.array(case["samples_bits"], np.uint32).view(np.float32)
+    for j, v in zip(range(0, len(lines), case["sample_stride"]), samples):
+        assert lines[j] == "%.7E" % float(v), j
