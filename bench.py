@@ -22,6 +22,8 @@ from the reference sources) or, when that library is absent, the oracle's C
 restatement, timed on this host's cores on a bounded sample of the same grid.
 
 Other workloads (--workload): plain512_f64, tb768_f32 (temporal blocking),
+plain16384_2d / tb16384_2d (the diffusion/ family's 2D 5-point operator on a
+16384^2 FP32 grid, row-streaming kernels, depth 1 / 8),
 batched512_f32 / batched512_bf16 (512 uneven grids per run, solves/s against
 launching the plain kernel per request) and served (request server: solves/s,
 p50/p99 latency under Poisson arrivals).  Each prints its own JSON line.
@@ -46,7 +48,43 @@ WORKLOADS = {
     "plain512_f32": (512, 512, 512, "f32", 1, 1000),
     "plain512_f64": (512, 512, 512, "f64", 1, 1000),
     "tb768_f32": (768, 768, 768, "f32", 4, 2000),
+    # 2D 5-point operator (nz = 1): diffusion/diffusion_openmp.cc:59-82
+    "plain16384_2d": (16384, 16384, 1, "f32", 1, 1000),
+    "tb16384_2d": (16384, 16384, 1, "f32", 8, 2000),
 }
+# cell updates of the CPU baseline sample per --cpu-steps step (the 512^3 grid)
+CPU_SAMPLE_CELLS = 512 ** 3
+
+
+def cpu_baseline_2d(nx, ny, steps_sample):
+    """Time the diffusion/ family's OpenMP RunKernel in 2D (oracle/_ref/libref2d.so)
+    or the oracle's 2D restatement on the host."""
+    gomp = ctypes.CDLL("libgomp.so.1")
+    gomp.omp_get_max_threads.restype = ctypes.c_int
+    threads = gomp.omp_get_max_threads()
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref2d.so")
+    if os.path.exists(ref):
+        L = ctypes.CDLL(ref)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.ref2d_run.argtypes = [ctypes.c_int] * 6 + [fp, fp, fp, fp, ctypes.POINTER(ctypes.c_double)]
+        sec = ctypes.c_double()
+        if L.ref2d_run(1, 2, nx, ny, 1, steps_sample, None, None, None, None, ctypes.byref(sec)) != 0:
+            raise RuntimeError("ref2d_run failed")
+        seconds, kind = sec.value, "reference"
+    else:
+        import oracle
+        k = oracle.constants2d_f32(nx, ny)
+        f = oracle.initialize2d_f32(nx, ny, k)
+        g = np.empty_like(f)
+        c = np.ascontiguousarray(k[8:15])
+        t0 = time.perf_counter()
+        oracle.lib().oracle_run2d_f32(f.ctypes.data_as(oracle._f), g.ctypes.data_as(oracle._f), nx, ny,
+                                      c.ctypes.data_as(oracle._f), steps_sample)
+        seconds, kind = time.perf_counter() - t0, "port"
+    return {"value": nx * ny * steps_sample / seconds / 1e9, "unit": "Gcell-updates/s",
+            "cores": threads, "kind": kind,
+            "sample": "%dx%d FP32 2D, %d steps, OpenMP RunKernel wall clock (%.2f s)"
+                      % (nx, ny, steps_sample, seconds)}
 
 
 def cpu_baseline(nx, ny, nz, steps_sample):
@@ -293,26 +331,37 @@ def main():
         tuned = ctx.autotune(nx, ny, nz, dtype, depth, arith)
         print("tuned:", tuned, file=sys.stderr, flush=True)
 
-    if dtype == d3.F32:
+    two_d = nz == 1
+    if two_d:
+        # the diffusion/ family's 2D benchmark: rows reversed on the device so
+        # the y-1 term is summed first, ct = cb = 0
+        k = d3.constants2d_f32(nx, ny)
+
+        def initial():
+            return d3.initialize2d_f32(nx, ny, k)
+    elif dtype == d3.F32:
         k = d3.constants_f32(nx, ny, nz)
-        f0 = d3.initialize_f32(nx, ny, nz, k)
+
+        def initial():
+            return d3.initialize_f32(nx, ny, nz, k)
     else:
         k = d3.constants_f64(nx, ny, nz)
-        f0 = d3.initialize_f64(nx, ny, nz, k)
+
+        def initial():
+            return d3.initialize_f64(nx, ny, nz, k)
     coef = d3.coefficients(k)
     field = d3.Field(ctx, nx, ny, nz, dtype)
-    field.upload(f0)
-    del f0
+    if two_d:
+        field.set_y_reversed(True)
+    field.upload(initial())
     if args.warmup > 0:
         field.run(args.warmup, coef, depth=depth, arith=arith, graph=False)
     # re-initialise after the warm-up (diffusion/diffusion.h:126-131)
-    field.upload(d3.initialize_f32(nx, ny, nz, k) if dtype == d3.F32 else
-                 d3.initialize_f64(nx, ny, nz, k))
+    field.upload(initial())
     if args.graph:
         # capture once (untimed), restore the input, then time a replay
         field.run(steps, coef, depth=depth, arith=arith, graph=True)
-        field.upload(d3.initialize_f32(nx, ny, nz, k) if dtype == d3.F32 else
-                     d3.initialize_f64(nx, ny, nz, k))
+        field.upload(initial())
     t0 = time.perf_counter()
     tm = field.run(steps, coef, depth=depth, arith=arith, graph=bool(args.graph))
     wall = time.perf_counter() - t0
@@ -320,8 +369,12 @@ def main():
 
     cells = nx * ny * nz
     gcells = cells * steps / wall / 1e9
-    err = field.error_analytical(float(k[0]), float(steps) * float(k[7]),
-                                 [float(v) for v in k[4:7]], [float(v) for v in k[1:4]])
+    wave = [float(v) for v in k[4:7]]
+    if two_d:
+        # no z factor in 2D: kz = pi with dz = 1 gives cos(kz * 0.5) = 0, factor 1
+        wave[2] = float(np.pi)
+    err = field.error_analytical(float(k[0]), float(steps) * float(k[7]), wave,
+                                 [float(v) for v in k[1:4]])
     field.close()
 
     bytes_per_launch = 2.0 * esize * cells          # one read + one write per cell
@@ -347,11 +400,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(nx if dtype == d3.F32 else nx, ny, nz, args.cpu_steps)
+        if two_d:
+            cpu = cpu_baseline_2d(nx, ny, max(1, args.cpu_steps * CPU_SAMPLE_CELLS // (nx * ny)))
+        else:
+            cpu = cpu_baseline(nx, ny, nz, args.cpu_steps)
 
-    metric = "Gcell-updates/s + %%HBM roofline at %d^3" % nx
+    size = ("%d^2 (2D 5-point)" % nx) if two_d else ("%d^3" % nx)
+    metric = "Gcell-updates/s + %%HBM roofline at %s" % size
     if depth > 1:
-        metric = "effective Gcell-updates/s + %%HBM roofline at %d^3, temporal depth %d" % (nx, depth)
+        metric = "effective Gcell-updates/s + %%HBM roofline at %s, temporal depth %d" % (size, depth)
     line = {
         "metric": metric,
         "value": round(gcells, 3),
@@ -365,8 +422,9 @@ def main():
         "vs_baseline": None,
         "dtype": dname,
         "data": "synthetic (analytical cosine-product field, reference coefficients)",
-        "config": {"workload": "%dx%dx%d %s 7-point diffusion, %d steps, depth %d, %s arithmetic"
-                               % (nx, ny, nz, dname.upper(), steps, depth, args.arith),
+        "config": {"workload": "%dx%dx%d %s %s diffusion, %d steps, depth %d, %s arithmetic"
+                               % (nx, ny, nz, dname.upper(), "2D 5-point" if two_d else "7-point",
+                                  steps, depth, args.arith),
                    "grid": [nx, ny, nz], "temporal_depth": depth, "arith": args.arith,
                    "graph": bool(args.graph), "tile": tm["tile"], "zchunks": tm["zchunks"],
                    "blocks": tm["blocks"], "parallelism": "single GPU"},

@@ -47,7 +47,7 @@ class Timing(ctypes.Structure):
 class TileDesc(ctypes.Structure):
     _fields_ = [("dtype", c_int), ("temporal_depth", c_int), ("arith", c_int), ("groups", c_int),
                 ("rows_per_wave", c_int), ("waves", c_int), ("tile_x", c_int), ("tile_y", c_int),
-                ("prefetch", c_int)]
+                ("prefetch", c_int), ("row_stream", c_int)]
 
 
 class Problem(ctypes.Structure):

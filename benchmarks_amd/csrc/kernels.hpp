@@ -26,6 +26,9 @@ struct KernelEntry {
   LaunchFn launch;
   const void *batch_func;      // nullptr when no batched instantiation exists
   LaunchBatchFn launch_batch;
+  // 1: row-streaming entry (one wave, one row per streamed "plane"); it runs
+  // nz == 1 fields as a stream over their rows and nothing else.
+  int rows = 0;
   int tile_x() const { return 64 * G * vec; }
   int tile_y() const { return NW * RPW; }
 };
@@ -63,11 +66,19 @@ void launch_stencil_batch(const BatchMember *members, const int *block_member, i
         &launch_stencil_batch<S, VEC, T, G, RPW, NW, FMA, 1>                                  \
   }
 
+// Row-streaming kernel (RPW = NW = 1).
+#define D3H_ENTRY_ROW(S, VEC, T, G, FMA, PF)                                                  \
+  KernelEntry {                                                                               \
+    S, VEC, T, G, 1, 1, FMA, PF, (const void *)&stencil_kernel<S, VEC, T, G, 1, 1, FMA, PF>,  \
+        &launch_stencil<S, VEC, T, G, 1, 1, FMA, PF>, nullptr, nullptr, 1                     \
+  }
+
 // Each kernels_*.hip file defines one table.
 const KernelEntry *kernels_f32_plain(int *n);
 const KernelEntry *kernels_f32_tb(int *n);
 const KernelEntry *kernels_f64(int *n);
 const KernelEntry *kernels_bf16(int *n);
 const KernelEntry *kernels_f16(int *n);
+const KernelEntry *kernels_f32_rows(int *n);
 
 }  // namespace d3h

@@ -48,7 +48,7 @@ const std::vector<KernelEntry> &registry() {
   static const std::vector<KernelEntry> all = [] {
     std::vector<KernelEntry> v;
     const KernelEntry *(*tables[])(int *) = {kernels_f32_plain, kernels_f32_tb, kernels_f64,
-                                              kernels_bf16, kernels_f16};
+                                              kernels_bf16, kernels_f16, kernels_f32_rows};
     for (auto fn : tables) {
       int n = 0;
       const KernelEntry *t = fn(&n);
@@ -59,9 +59,9 @@ const std::vector<KernelEntry> &registry() {
   return all;
 }
 
-const KernelEntry *default_entry(int dtype, int T, int fma) {
+const KernelEntry *default_entry(int dtype, int T, int fma, bool rows) {
   for (const auto &e : registry())
-    if (e.dtype == dtype && e.T == T && e.fma == fma) return &e;
+    if (e.dtype == dtype && e.T == T && e.fma == fma && (e.rows != 0) == rows) return &e;
   return nullptr;
 }
 
@@ -137,8 +137,52 @@ static const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, 
   return nullptr;
 }
 
+bool rows_ok(const d3h_field *f, const double coef[7]) {
+  return f->nz == 1 && coef[4] == 0.0 && coef[5] == 0.0;
+}
+
+// Row streaming.  An nx x ny x 1 field is run as a stream over its rows: the
+// tile runner sees one-row planes (ny' = 1, nz' = ny) walked from the last
+// row to the first (plane stride -pitch), so its z-1 / z+1 neighbours are
+// rows y+1 / y-1.  With coefficients (ce, cw, 0, 0, cn, cs, cc) in the
+// (ce cw cn cs ct cb cc) slots it evaluates
+//   cc c + cw w + ce e + 0 c + 0 c + cs f[y+1] + cn f[y-1],
+// the 7-point step of the nz == 1 field (whose clamped z terms are 0 c when
+// ct == cb == 0) with the y terms in the same order; adding the zero terms
+// before instead of after the y terms changes at most the sign of an exactly
+// zero result.
+int field_geometry(const KernelEntry &e, const d3h_field *f, int zchunks, Geometry *g,
+                   unsigned *blocks) {
+  if (!e.rows)
+    return make_geometry(e, f->nx, f->ny, f->nz, f->pitch, zchunks, f->ctx->num_cus, occupancy_of(e), g,
+                         blocks);
+  if (f->nz != 1) return fail(D3H_ERR_INVALID, "row-streaming tile on a field with nz = %d", f->nz);
+  const int occ = occupancy_of(e), cus = f->ctx->num_cus;
+  if (zchunks <= 0) {
+    // one full wave of resident workgroups, chunks of at least max(16T, 64)
+    // rows (a row is cheap, so the 2T halo rows must stay a small share)
+    D3H_TRY(make_geometry(e, f->nx, 1, f->ny, f->pitch, 1, cus, occ, g, blocks));
+    const long long slots = (long long)std::max(occ, 1) * std::max(cus, 1);
+    zchunks = (int)std::max(1LL, std::min(slots / g->ntx, (long long)(f->ny / std::max(16 * e.T, 64))));
+  }
+  D3H_TRY(make_geometry(e, f->nx, 1, f->ny, f->pitch, zchunks, cus, occ, g, blocks));
+  g->plane = -(long long)f->pitch;
+  return D3H_OK;
+}
+
+static void launch_entry(const KernelEntry &e, const d3h_field *f, const void *src, void *dst,
+                         const Geometry &g, const double coef[7], unsigned blocks, hipStream_t stream) {
+  if (!e.rows) {
+    e.launch(src, dst, g, coef, blocks, stream);
+    return;
+  }
+  const size_t last = (size_t)(f->ny - 1) * f->pitch * dtype_bytes(f->dtype);
+  const double rc[7] = {coef[0], coef[1], 0.0, 0.0, coef[2], coef[3], coef[6]};
+  e.launch(static_cast<const char *>(src) + last, static_cast<char *>(dst) + last, g, rc, blocks, stream);
+}
+
 int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
-                  const KernelEntry **entry, int *zchunks) {
+                  bool rows, const KernelEntry **entry, int *zchunks) {
   const int T = cfg.temporal_depth <= 0 ? 1 : cfg.temporal_depth;
   const int fma = cfg.arith == D3H_ARITH_FMA ? 1 : 0;
   if (cfg.arith != D3H_ARITH_EXACT && cfg.arith != D3H_ARITH_FMA)
@@ -150,6 +194,8 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
     if (e.dtype != dtype || e.T != T || e.fma != fma)
       return fail(D3H_ERR_INVALID, "tile %d is dtype %d depth %d fma %d, run wants %d/%d/%d", cfg.tile,
                   e.dtype, e.T, e.fma, dtype, T, fma);
+    if (e.rows && !rows)
+      return fail(D3H_ERR_INVALID, "tile %d streams rows: it needs nz == 1 and ct == cb == 0", cfg.tile);
     *entry = &e;
     *zchunks = cfg.zchunks;
     return D3H_OK;
@@ -160,14 +206,15 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
     if (it != ctx->tuned.end()) {
       const KernelEntry *e = find_entry(dtype, T, fma, it->second.vec, it->second.G, it->second.RPW,
                                         it->second.NW, it->second.pf);
-      if (e) {
+      if (e && (!e->rows || rows)) {
         *entry = e;
         *zchunks = cfg.zchunks > 0 ? cfg.zchunks : it->second.zchunks;
         return D3H_OK;
       }
     }
   }
-  const KernelEntry *e = default_entry(dtype, T, fma);
+  const KernelEntry *e = rows ? default_entry(dtype, T, fma, true) : nullptr;
+  if (!e) e = default_entry(dtype, T, fma);
   if (!e) return fail(D3H_ERR_UNSUPPORTED, "no kernel for dtype %d depth %d fma %d", dtype, T, fma);
   *entry = e;
   *zchunks = cfg.zchunks;
@@ -180,22 +227,22 @@ int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEnt
   const int nfull = count / T, rem = count % T;
   Geometry g{}, g1{};
   unsigned nb = 0, nb1 = 0;
-  D3H_TRY(make_geometry(e, f->nx, f->ny, f->nz, f->pitch, zchunks, f->ctx->num_cus, occupancy_of(e),
-                        &g, &nb));
+  if (e.rows && !rows_ok(f, coef))
+    return fail(D3H_ERR_INVALID, "row-streaming tile needs nz == 1 and ct == cb == 0");
+  D3H_TRY(field_geometry(e, f, zchunks, &g, &nb));
   const KernelEntry *plain = nullptr;
   if (rem) {
-    plain = default_entry(f->dtype, 1, e.fma);
+    plain = default_entry(f->dtype, 1, e.fma, e.rows != 0);
     if (!plain) return fail(D3H_ERR_UNSUPPORTED, "no plain kernel for dtype %d", f->dtype);
-    D3H_TRY(make_geometry(*plain, f->nx, f->ny, f->nz, f->pitch, 0, f->ctx->num_cus,
-                          occupancy_of(*plain), &g1, &nb1));
+    D3H_TRY(field_geometry(*plain, f, 0, &g1, &nb1));
   }
   int n = 0;
   for (int i = 0; i < nfull; ++i, ++n) {
-    e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g, coef, nb, stream);
+    launch_entry(e, f, f->buf[f->cur], f->buf[f->cur ^ 1], g, coef, nb, stream);
     f->cur ^= 1;
   }
   for (int i = 0; i < rem; ++i, ++n) {
-    plain->launch(f->buf[f->cur], f->buf[f->cur ^ 1], g1, coef, nb1, stream);
+    launch_entry(*plain, f, f->buf[f->cur], f->buf[f->cur ^ 1], g1, coef, nb1, stream);
     f->cur ^= 1;
   }
   D3H_HIP(hipGetLastError());
@@ -586,7 +633,7 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
   D3H_HIP(hipSetDevice(ctx->device));
   const KernelEntry *e = nullptr;
   int zchunks = 0;
-  D3H_TRY(resolve_entry(ctx, f->nx, f->ny, f->nz, f->dtype, cfg, &e, &zchunks));
+  D3H_TRY(resolve_entry(ctx, f->nx, f->ny, f->nz, f->dtype, cfg, rows_ok(f, coef), &e, &zchunks));
   int launches = 0;
   unsigned blocks = 0;
   float ms = 0.f;
@@ -641,8 +688,7 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
     timing->tile = (int)(e - registry().data());
     Geometry g{};
     unsigned nb = 0;
-    if (make_geometry(*e, f->nx, f->ny, f->nz, f->pitch, zchunks, ctx->num_cus, occupancy_of(*e), &g,
-                      &nb) == D3H_OK) {
+    if (field_geometry(*e, f, zchunks, &g, &nb) == D3H_OK) {
       timing->zchunks = g.nzc;
       timing->blocks = (int)nb;
     }
@@ -734,6 +780,7 @@ int d3h_tile_info(int index, d3h_tile_desc *out) {
   out->tile_x = e.tile_x();
   out->tile_y = e.tile_y();
   out->prefetch = e.pf;
+  out->row_stream = e.rows;
   return D3H_OK;
 }
 
@@ -744,7 +791,10 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
   const int fma = arith == D3H_ARITH_FMA ? 1 : 0;
   d3h_field *f = nullptr;
   D3H_TRY(d3h_field_create(ctx, nx, ny, nz, dtype, &f));
-  const double coef[7] = {0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4};
+  // nz == 1 fields are tuned for the 2D operator (ct = cb = 0), which also
+  // admits the row-streaming tiles
+  const bool rows = nz == 1;
+  const double coef[7] = {0.1, 0.1, 0.1, 0.1, rows ? 0.0 : 0.1, rows ? 0.0 : 0.1, rows ? 0.6 : 0.4};
   const auto &reg = registry();
   double best = 1e300;
   int bi = -1, bz = 0;
@@ -754,26 +804,27 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
   int rc = D3H_OK;
   for (int i = 0; i < (int)reg.size() && rc == D3H_OK; ++i) {
     const KernelEntry &e = reg[i];
-    if (e.dtype != dtype || e.T != T || e.fma != fma) continue;
+    if (e.dtype != dtype || e.T != T || e.fma != fma || (e.rows && !rows)) continue;
     const int occ = occupancy_of(e);
     if (occ <= 0) continue;
     Geometry g{};
     unsigned nb = 0;
-    if (make_geometry(e, nx, ny, nz, f->pitch, 0, ctx->num_cus, occ, &g, &nb) != D3H_OK) continue;
+    if (field_geometry(e, f, 0, &g, &nb) != D3H_OK) continue;
+    const int nstream = e.rows ? ny : nz;  // planes (rows) streamed
     // few long streams (1-4 chunks) up to several waves of workgroups
     std::vector<int> cands = {1, 2, 3, 4, g.nzc, g.nzc * 2, g.nzc * 4, std::max(1, g.nzc / 2)};
     std::sort(cands.begin(), cands.end());
     cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
     for (int zc : cands) {
-      if (zc > nz) continue;
+      if (zc > nstream) continue;
       Geometry g2{};
       unsigned nb2 = 0;
-      if (make_geometry(e, nx, ny, nz, f->pitch, zc, ctx->num_cus, occ, &g2, &nb2) != D3H_OK) continue;
-      e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);  // warm-up
+      if (field_geometry(e, f, zc, &g2, &nb2) != D3H_OK) continue;
+      launch_entry(e, f, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);  // warm-up
       f->cur ^= 1;
       (void)hipEventRecord(ctx->ev0, ctx->stream);
       for (int r = 0; r < reps; ++r) {
-        e.launch(f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);
+        launch_entry(e, f, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);
         f->cur ^= 1;
       }
       (void)hipEventRecord(ctx->ev1, ctx->stream);

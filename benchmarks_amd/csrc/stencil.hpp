@@ -71,7 +71,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, in
 //   [ty*oy, ty*oy+oy), planes [tz*oz, tz*oz+oz) (clipped to the grid) and loads
 //   a window that extends hx cells / T rows / T planes beyond that on every
 //   side that is not a domain boundary.  pitch is the padded row length
-//   (multiple of 16 bytes); plane = pitch * ny.
+//   (multiple of 16 bytes); plane = pitch * ny, or -pitch when the rows of an
+//   nz == 1 field are streamed as planes from the last row to the first.
 struct Geometry {
   int nx, ny, nz, pitch;
   long long plane;
@@ -288,7 +289,8 @@ struct TileRunner {
     zl1 = min(zo1 + T, geo.nz);
     edge_xy = (xl0 + TILE_X > geo.nx) || (yl0 + TILE_Y > geo.ny);
     xlane = xl0 + lane * VEC;
-    plane_bytes = (int)(geo.plane * (long long)sizeof(ST));
+    // a negative plane stride walks the planes downwards (row streaming)
+    plane_bytes = (int)((geo.plane < 0 ? -geo.plane : geo.plane) * (long long)sizeof(ST));
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int y = yl0 + wave * RPW + r;

@@ -38,7 +38,7 @@ SAMPLE_STRIDE = 997
 def _lib():
     lib = ctypes.CDLL(LIB)
     fp = ctypes.POINTER(ctypes.c_float)
-    lib.ref2d_run.argtypes = [ctypes.c_int] * 6 + [fp, fp, fp, fp]
+    lib.ref2d_run.argtypes = [ctypes.c_int] * 6 + [fp, fp, fp, fp, ctypes.POINTER(ctypes.c_double)]
     lib.ref2d_run.restype = ctypes.c_int
     return lib
 
@@ -52,7 +52,7 @@ def run(lib, variant, nd, shape, count):
     out = np.empty(nz * ny * nx, dtype=np.float32)
     acc = np.zeros(1, dtype=np.float32)
     const = np.zeros(15, dtype=np.float32)
-    rc = lib.ref2d_run(variant, nd, nx, ny, nz, count, None, _ptr(out), _ptr(acc), _ptr(const))
+    rc = lib.ref2d_run(variant, nd, nx, ny, nz, count, None, _ptr(out), _ptr(acc), _ptr(const), None)
     assert rc == 0
     return out, float(acc[0]), const
 

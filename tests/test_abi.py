@@ -70,12 +70,17 @@ def test_tile_table():
         assert t["dtype"] in (0, 1, 2, 3) and t["temporal_depth"] in (1, 2, 4, 8)
         assert t["tile_y"] == t["rows_per_wave"] * t["waves"]
         assert t["tile_x"] % 64 == 0
-        assert t["tile_y"] > 2 * t["temporal_depth"] or t["temporal_depth"] == 1
+        if t["row_stream"]:               # streams the rows of nz == 1 fields
+            assert t["tile_y"] == 1 and t["dtype"] == 0
+        else:
+            assert t["tile_y"] > 2 * t["temporal_depth"] or t["temporal_depth"] == 1
     for dt in (0, 1, 2, 3):
         for T in (1, 2, 4):
             for ar in (0, 1):
                 assert any(t["dtype"] == dt and t["temporal_depth"] == T and t["arith"] == ar
-                           for t in tl), (dt, T, ar)
+                           and not t["row_stream"] for t in tl), (dt, T, ar)
+    for T in (1, 2, 4, 8):
+        assert any(t["row_stream"] and t["temporal_depth"] == T for t in tl), T
     with pytest.raises(d3.D3HError):
         N.check(N.lib().d3h_tile_info(len(tl), ctypes.byref(N.TileDesc())), "tile_info")
 

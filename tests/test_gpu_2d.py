@@ -150,3 +150,81 @@ def test_driver_nd2_matches_family_output(golden2d, tmp_path):
     samples = np.array(case["samples_bits"], np.uint32).view(np.float32)
     for j, v in zip(range(0, len(lines), case["sample_stride"]), samples):
         assert lines[j] == "%.7E" % float(v), j
+
+
+def row_tiles(T=None, arith=d3.ARITH_EXACT):
+    return [t for t in d3.tiles() if t["row_stream"] and t["arith"] == arith
+            and (T is None or t["temporal_depth"] == T)]
+
+
+@pytest.mark.parametrize("shape", [(300, 77), (1000, 9), (5, 300), (520, 1), (1, 40)])
+def test_every_row_tile_exact_matches_oracle(ctx, shape):
+    """Row-streaming tiles (every depth, chunking) against the oracle's 2D
+    operator on noisy fields, counts that are not multiples of the depth."""
+    nx, ny = shape
+    k = d3.constants2d_f32(nx, ny)
+    f0 = d3.add_noise_f32(d3.initialize2d_f32(nx, ny, k).reshape(-1), 3, 1e-3).reshape(ny, nx)
+    want = {c: oracle.run2d_f32(f0, k[8:15], c) for c in (5, 11)}
+    c = d3.coefficients(k)
+    c[4] = c[5] = 0.0
+    for t in row_tiles():
+        T = t["temporal_depth"]
+        for count in (5, 11):
+            for zc in (0, 1, 3):
+                print("row tile", t["index"], "T", T, shape, count, zc, flush=True)
+                with d3.Field(ctx, nx, ny, 1) as f:
+                    f.set_y_reversed(True)
+                    f.upload(f0)
+                    tm = f.run(count, c, depth=T, tile=t["index"], zchunks=zc)
+                    out = f.download().reshape(ny, nx)
+                assert tm["steps"] == count
+                assert np.array_equal(out.view(np.uint32), want[count].view(np.uint32)), (t, count, zc)
+
+
+def test_row_tiles_fma_within_tolerance(ctx):
+    nx, ny = 700, 300
+    k = d3.constants2d_f32(nx, ny)
+    f0 = d3.initialize2d_f32(nx, ny, k)
+    want = oracle.run2d_f32(f0, k[8:15], 40)
+    c = d3.coefficients(k)
+    c[4] = c[5] = 0.0
+    for t in row_tiles(arith=d3.ARITH_FMA):
+        out = run2d(ctx, f0, c, 40, depth=t["temporal_depth"], tile=t["index"], arith=d3.ARITH_FMA)
+        rel, _, _ = oracle.diff_norms(out, want)
+        assert rel < 1e-6, (t, rel)
+
+
+def test_auto_choice_is_row_streaming_and_exact(ctx):
+    nx, ny = 2048, 1024
+    k = d3.constants2d_f32(nx, ny)
+    f0 = d3.initialize2d_f32(nx, ny, k)
+    c = d3.coefficients(k)
+    for depth in (1, 2, 4, 8):
+        with d3.Field(ctx, nx, ny, 1) as f:
+            f.set_y_reversed(True)
+            f.upload(f0)
+            tm = f.run(13, c, depth=depth)
+            out = f.download().reshape(ny, nx)
+        assert d3.tiles()[tm["tile"]]["row_stream"] == 1, tm
+        assert np.array_equal(out.view(np.uint32), oracle.run2d_f32(f0, k[8:15], 13).view(np.uint32))
+
+
+def test_row_tiles_refuse_3d_operator(ctx):
+    """A row tile is only for nz == 1 with ct == cb == 0; nz == 1 fields with
+    t/b coefficients run on the 3D tiles and match the 7-point oracle."""
+    t = row_tiles(1)[0]
+    with d3.Field(ctx, 64, 16, 4) as f:
+        f.upload(np.ones((4, 16, 64), np.float32))
+        with pytest.raises(d3.D3HError):
+            f.run(1, [0.1] * 6 + [0.4], tile=t["index"])
+    rng = np.random.default_rng(2)
+    f0 = rng.random((1, 40, 300), dtype=np.float32)
+    coef = np.array([0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4], np.float32)
+    with d3.Field(ctx, 300, 40, 1) as f:
+        f.upload(f0)
+        with pytest.raises(d3.D3HError):
+            f.run(1, coef, tile=t["index"])
+        tm = f.run(6, coef)
+        assert d3.tiles()[tm["tile"]]["row_stream"] == 0
+        out = f.download()
+    assert np.array_equal(out.view(np.uint32), oracle.run_f32(f0, coef, 6).view(np.uint32))
