@@ -97,7 +97,8 @@ struct Request {
   int pitch = 0;
   int cur = 0;
   int remaining = 0;
-  bool freed = false;
+  bool freed = false;     // pool memory returned (scheduler)
+  bool released = false;  // record released by d3h_server_wait before `freed`
 };
 
 }  // namespace
@@ -207,6 +208,7 @@ int round_once(d3h_server *s, bool *did_work) {
         s->alloc.release(r->off, r->footprint);
         r->freed = true;
         it = s->draining.erase(it);
+        if (r->released) delete r;  // its waiter has already returned
       } else {
         ++it;
       }
@@ -397,6 +399,9 @@ int d3h_server_destroy(d3h_server *s) {
   if (s->compute) (void)hipStreamDestroy(s->compute);
   if (s->copy) (void)hipStreamDestroy(s->copy);
   if (s->pool) (void)hipFree(s->pool);
+  // released records still draining are no longer in `reqs`
+  for (Request *r : s->draining)
+    if (r->released) delete r;
   for (auto &kv : s->reqs) delete kv.second;
   delete s;
   return D3H_OK;
@@ -462,8 +467,16 @@ int d3h_server_wait(d3h_server *s, int64_t id, double *latency_ms) {
     const int st = r->state.load();
     return st == kDone || st == kFailed;
   });
-  if (latency_ms) *latency_ms = r->state.load() == kDone ? r->t_done - r->t_submit : 0.0;
-  if (r->state.load() == kFailed) return fail(r->status, "request %lld failed", (long long)id);
+  if (r->state.load() == kFailed) {
+    if (latency_ms) *latency_ms = 0.0;
+    return fail(r->status, "request %lld failed", (long long)id);
+  }
+  if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+  // release the record: at once when its memory is back in the pool, else
+  // when the scheduler returns it (the record must outlive its draining)
+  s->reqs.erase(it);
+  if (r->freed) delete r;
+  else r->released = true;
   return D3H_OK;
 }
 

@@ -233,8 +233,14 @@ int d3h_server_destroy(d3h_server *s);
  * until the request is done.  Returns the request id in *id. */
 int d3h_server_submit(d3h_server *s, const d3h_problem *p, const void *host_in,
                       int host_dtype, void *host_out, int out_dtype, int64_t *id);
-/* done = 1 when finished; latency_ms from submit to result-in-host-memory. */
+/* done = 1 when finished; latency_ms from submit to result-in-host-memory.
+ * Polling never releases the request's record. */
 int d3h_server_poll(d3h_server *s, int64_t id, int *done, double *latency_ms);
+/* Blocks until the request is finished.  A successful wait releases the
+ * request's record: later calls with its id return D3H_ERR_NOT_FOUND, so wait
+ * exactly once per request to keep a long-running server's memory bounded.
+ * A failed request keeps its record (and reports its error) until the server
+ * is destroyed. */
 int d3h_server_wait(d3h_server *s, int64_t id, double *latency_ms);
 int d3h_server_stats(d3h_server *s, int64_t *completed, int64_t *rejected,
                      int64_t *sweeps, double *busy_ms);

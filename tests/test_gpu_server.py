@@ -10,6 +10,7 @@ import pytest
 
 import benchmarks_amd as d3
 import oracle
+from benchmarks_amd import _native as N
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +52,10 @@ def test_server_results_equal_alone(ctx, depth, sweep):
             t.join()
         lats = [srv.wait(i) for i in ids]
         st = srv.stats()
+        # a successful wait releases the request's record
+        with pytest.raises(d3.D3HError) as e:
+            srv.poll(ids[0])
+        assert e.value.status == N.D3H_ERR_NOT_FOUND
     assert st["completed"] == len(reqs) and st["rejected"] == 0
     assert all(l > 0 for l in lats)
     for i, (p, f0) in enumerate(reqs):
