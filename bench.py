@@ -17,6 +17,12 @@ duration from those events, against the 8 TB/s HBM peak.  traffic is read
 from profiles/traffic_<workload>.json (rocprofv3 FETCH_SIZE / WRITE_SIZE),
 when present.
 
+error_vs_fp64_gpu_oracle: relative L2 and max-abs difference of the FP32
+result from the same initial field and coefficients advanced in FP64 on the
+GPU (EXACT arithmetic); error_vs_analytical_fp64: RMS / max-abs against the
+exact PDE solution, reduced in FP64 on the device.  tb768_f32 starts from the
+cosine product plus seeded uniform noise of amplitude 1e-3.
+
 cpu_baseline: the reference's OpenMP RunKernel (oracle/_ref/libref.so, built
 from the reference sources) or, when that library is absent, the oracle's C
 restatement, timed on this host's cores on a bounded sample of the same grid.
@@ -44,14 +50,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 WORKLOADS = {
-    # name: (nx, ny, nz, dtype, depth, default steps)
-    "plain512_f32": (512, 512, 512, "f32", 1, 1000),
-    "plain512_f64": (512, 512, 512, "f64", 1, 1000),
-    "tb768_f32": (768, 768, 768, "f32", 4, 2000),
+    # name: (nx, ny, nz, dtype, depth, default steps, seeded noise amplitude)
+    "plain512_f32": (512, 512, 512, "f32", 1, 1000, 0.0),
+    "plain512_f64": (512, 512, 512, "f64", 1, 1000, 0.0),
+    "tb768_f32": (768, 768, 768, "f32", 4, 2000, 1e-3),
     # 2D 5-point operator (nz = 1): diffusion/diffusion_openmp.cc:59-82
-    "plain16384_2d": (16384, 16384, 1, "f32", 1, 1000),
-    "tb16384_2d": (16384, 16384, 1, "f32", 8, 2000),
+    "plain16384_2d": (16384, 16384, 1, "f32", 1, 1000, 0.0),
+    "tb16384_2d": (16384, 16384, 1, "f32", 8, 2000, 0.0),
 }
+NOISE_SEED = 768
 # cell updates of the CPU baseline sample per --cpu-steps step (the 512^3 grid)
 CPU_SAMPLE_CELLS = 512 ** 3
 
@@ -289,11 +296,13 @@ def main():
     ap.add_argument("--workload", default="plain512_f32",
                     choices=sorted(WORKLOADS) + ["batched512_f32", "batched512_bf16", "served"])
     ap.add_argument("--arith", default="exact", choices=["exact", "fma"])
-    ap.add_argument("--depth", type=int, default=None, help="temporal depth (default per workload)")
+    ap.add_argument("--depth", type=int, default=None,
+                    help="temporal depth (default per workload); 0 = the autotuner's choice")
     ap.add_argument("--graph", type=int, default=1, help="replay a captured hipGraph (1) or launch")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile before timing")
     ap.add_argument("--cpu-steps", type=int, default=500, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU oracle comparison")
     ap.add_argument("--requests", type=int, default=150, help="served: requests per run")
     ap.add_argument("--rates", default="50,100,150", help="served: Poisson arrival rates (1/s)")
     ap.add_argument("--sweep-steps", type=int, default=32, help="served: steps per scheduling round")
@@ -313,7 +322,7 @@ def main():
         print(json.dumps(line), flush=True)
         ctx.close()
         return 0
-    nx, ny, nz, dname, depth, dsteps = WORKLOADS[args.workload]
+    nx, ny, nz, dname, depth, dsteps, noise = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else dsteps
     depth = args.depth if args.depth is not None else depth
     dtype = {"f32": d3.F32, "f64": d3.F64}[dname]
@@ -343,7 +352,8 @@ def main():
         k = d3.constants_f32(nx, ny, nz)
 
         def initial():
-            return d3.initialize_f32(nx, ny, nz, k)
+            f = d3.initialize_f32(nx, ny, nz, k)
+            return d3.add_noise_f32(f, NOISE_SEED, noise) if noise else f
     else:
         k = d3.constants_f64(nx, ny, nz)
 
@@ -375,6 +385,18 @@ def main():
         wave[2] = float(np.pi)
     err = field.error_analytical(float(k[0]), float(steps) * float(k[7]), wave,
                                  [float(v) for v in k[1:4]])
+    # the FP64 GPU oracle: the same initial field and coefficients advanced in
+    # FP64 (EXACT arithmetic; fused depths are bitwise equal to plain steps)
+    err64 = None
+    if dtype == d3.F32 and not args.no_fp64:
+        t64 = time.perf_counter()
+        with d3.Field(ctx, nx, ny, nz, d3.F64) as ref:
+            if two_d:
+                ref.set_y_reversed(True)
+            ref.upload(initial())
+            ref.run(steps, coef, depth=1 if two_d else 4)
+            err64 = field.compare(ref)
+        print("fp64 oracle: %.1f s" % (time.perf_counter() - t64), file=sys.stderr, flush=True)
     field.close()
 
     bytes_per_launch = 2.0 * esize * cells          # one read + one write per cell
@@ -405,10 +427,12 @@ def main():
         else:
             cpu = cpu_baseline(nx, ny, nz, args.cpu_steps)
 
+    depth_used = tm["temporal_depth"]
     size = ("%d^2 (2D 5-point)" % nx) if two_d else ("%d^3" % nx)
     metric = "Gcell-updates/s + %%HBM roofline at %s" % size
-    if depth > 1:
-        metric = "effective Gcell-updates/s + %%HBM roofline at %s, temporal depth %d" % (size, depth)
+    if depth_used > 1:
+        metric = "effective Gcell-updates/s + %%HBM roofline at %s, temporal depth %d" % (size,
+                                                                                       depth_used)
     line = {
         "metric": metric,
         "value": round(gcells, 3),
@@ -421,11 +445,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dname,
-        "data": "synthetic (analytical cosine-product field, reference coefficients)",
+        "data": "synthetic (analytical cosine-product field%s, reference coefficients)"
+                % (", + seeded uniform noise amplitude %g" % noise if noise else ""),
         "config": {"workload": "%dx%dx%d %s %s diffusion, %d steps, depth %d, %s arithmetic"
                                % (nx, ny, nz, dname.upper(), "2D 5-point" if two_d else "7-point",
-                                  steps, depth, args.arith),
-                   "grid": [nx, ny, nz], "temporal_depth": depth, "arith": args.arith,
+                                  steps, depth_used, args.arith),
+                   "grid": [nx, ny, nz], "temporal_depth": depth_used,
+                   "depth_choice": "autotuned" if depth == 0 else "fixed", "arith": args.arith,
                    "graph": bool(args.graph), "tile": tm["tile"], "zchunks": tm["zchunks"],
                    "blocks": tm["blocks"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
@@ -437,6 +463,8 @@ def main():
         "copy_probe_gbps": round(probe["gbps"], 1),
         "frac_of_copy_probe": round(achieved / probe["gbps"], 4),
         "error_vs_analytical_fp64": {"rms": err["rms"], "max_abs": err["max_abs"]},
+        "error_vs_fp64_gpu_oracle": ({"rel_l2": err64["rel_l2"], "max_abs": err64["max_abs"]}
+                                     if err64 else None),
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -70,7 +70,8 @@ int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int f
   std::vector<int> qa(n, 0), rb(n, 0);
   for (int i = 0; i < n; ++i) {
     const PlanMember &m = ms[i];
-    int Tm = T;
+    // depth 0: the depth the autotuner chose for the member's size class, else 2
+    int Tm = T != 0 ? T : tuned_depth(ctx, m.dtype, fma, m.nx, m.ny, m.nz, 2);
     const KernelEntry *e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, Tm, fma);
     if (!e && Tm != 1) {
       Tm = 1;
@@ -325,8 +326,8 @@ int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph, d3
   const double t0 = now_ms();
   if (!b) return fail(D3H_ERR_INVALID, "batch is NULL");
   if (arith != D3H_ARITH_EXACT && arith != D3H_ARITH_FMA) return fail(D3H_ERR_INVALID, "arith %d", arith);
-  const int T = temporal_depth <= 0 ? 2 : temporal_depth;
-  if (T != 1 && T != 2 && T != 4 && T != 8) return fail(D3H_ERR_INVALID, "depth %d", T);
+  const int T = temporal_depth;
+  if (T != 0 && T != 1 && T != 2 && T != 4 && T != 8) return fail(D3H_ERR_INVALID, "depth %d", T);
   const int fma = arith == D3H_ARITH_FMA;
   d3h_context *ctx = b->ctx;
   D3H_HIP(hipSetDevice(ctx->device));

@@ -181,16 +181,25 @@ static void launch_entry(const KernelEntry &e, const d3h_field *f, const void *s
   e.launch(static_cast<const char *>(src) + last, static_cast<char *>(dst) + last, g, rc, blocks, stream);
 }
 
+int tuned_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, int dflt) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto it = ctx->tuned.find(tune_key(dtype, 0, fma, nx, ny, nz));
+  return it != ctx->tuned.end() ? it->second.T : dflt;
+}
+
 int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
                   bool rows, const KernelEntry **entry, int *zchunks) {
-  const int T = cfg.temporal_depth <= 0 ? 1 : cfg.temporal_depth;
+  int T = cfg.temporal_depth;
   const int fma = cfg.arith == D3H_ARITH_FMA ? 1 : 0;
   if (cfg.arith != D3H_ARITH_EXACT && cfg.arith != D3H_ARITH_FMA)
     return fail(D3H_ERR_INVALID, "unknown arith mode %d", cfg.arith);
+  if (T != 0 && T != 1 && T != 2 && T != 4 && T != 8)
+    return fail(D3H_ERR_INVALID, "temporal depth %d (0 = tuned, 1, 2, 4 or 8)", T);
   const auto &reg = registry();
   if (cfg.tile >= 0) {
     if (cfg.tile >= (int)reg.size()) return fail(D3H_ERR_INVALID, "tile index %d out of range", cfg.tile);
     const KernelEntry &e = reg[cfg.tile];
+    if (T == 0) T = e.T;  // an explicit tile brings its own depth
     if (e.dtype != dtype || e.T != T || e.fma != fma)
       return fail(D3H_ERR_INVALID, "tile %d is dtype %d depth %d fma %d, run wants %d/%d/%d", cfg.tile,
                   e.dtype, e.T, e.fma, dtype, T, fma);
@@ -202,6 +211,20 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
   }
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
+    if (T == 0) {
+      // the depth the autotuner chose for this size class, else plain steps
+      auto it = ctx->tuned.find(tune_key(dtype, 0, fma, nx, ny, nz));
+      if (it != ctx->tuned.end()) {
+        const TuneVal &v = it->second;
+        const KernelEntry *e = find_entry(dtype, v.T, fma, v.vec, v.G, v.RPW, v.NW, v.pf);
+        if (e && (!e->rows || rows)) {
+          *entry = e;
+          *zchunks = cfg.zchunks > 0 ? cfg.zchunks : v.zchunks;
+          return D3H_OK;
+        }
+      }
+      T = 1;
+    }
     auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz));
     if (it != ctx->tuned.end()) {
       const KernelEntry *e = find_entry(dtype, T, fma, it->second.vec, it->second.G, it->second.RPW,
@@ -628,7 +651,6 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
   d3h_run_config cfg;
   d3h_run_config_default(&cfg);
   if (cfg_in) cfg = *cfg_in;
-  if (cfg.temporal_depth <= 0) cfg.temporal_depth = 1;
   d3h_context *ctx = f->ctx;
   D3H_HIP(hipSetDevice(ctx->device));
   const KernelEntry *e = nullptr;
@@ -784,33 +806,24 @@ int d3h_tile_info(int index, d3h_tile_desc *out) {
   return D3H_OK;
 }
 
-int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int temporal_depth, int arith,
-                 int *best_tile, int *best_zchunks, double *best_ms_per_step) {
-  if (!ctx) return fail(D3H_ERR_INVALID, "ctx is NULL");
-  const int T = temporal_depth <= 0 ? 1 : temporal_depth;
-  const int fma = arith == D3H_ARITH_FMA ? 1 : 0;
-  d3h_field *f = nullptr;
-  D3H_TRY(d3h_field_create(ctx, nx, ny, nz, dtype, &f));
-  // nz == 1 fields are tuned for the 2D operator (ct = cb = 0), which also
-  // admits the row-streaming tiles
-  const bool rows = nz == 1;
-  const double coef[7] = {0.1, 0.1, 0.1, 0.1, rows ? 0.0 : 0.1, rows ? 0.0 : 0.1, rows ? 0.6 : 0.4};
+// Time every tile of depth T (and its z-chunk candidates) on field f; the
+// fastest per step goes to *bi / *bz / *best (bi < 0 when none ran).
+static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const double coef[7], int *bi,
+                      int *bz, double *best) {
   const auto &reg = registry();
-  double best = 1e300;
-  int bi = -1, bz = 0;
-  const long long cells = (long long)nx * ny * nz;
+  const bool rows = rows_ok(f, coef);
+  const long long cells = (long long)f->nx * f->ny * f->nz;
   // enough launches for ~2 ms of work at ~1 Tcell/s, at least 4
   const int reps = (int)std::max(4LL, std::min(200LL, 2000000000LL / std::max(cells, 1LL) / T));
-  int rc = D3H_OK;
-  for (int i = 0; i < (int)reg.size() && rc == D3H_OK; ++i) {
+  for (int i = 0; i < (int)reg.size(); ++i) {
     const KernelEntry &e = reg[i];
-    if (e.dtype != dtype || e.T != T || e.fma != fma || (e.rows && !rows)) continue;
+    if (e.dtype != f->dtype || e.T != T || e.fma != fma || (e.rows && !rows)) continue;
     const int occ = occupancy_of(e);
     if (occ <= 0) continue;
     Geometry g{};
     unsigned nb = 0;
     if (field_geometry(e, f, 0, &g, &nb) != D3H_OK) continue;
-    const int nstream = e.rows ? ny : nz;  // planes (rows) streamed
+    const int nstream = e.rows ? f->ny : f->nz;  // planes (rows) streamed
     // few long streams (1-4 chunks) up to several waves of workgroups
     std::vector<int> cands = {1, 2, 3, 4, g.nzc, g.nzc * 2, g.nzc * 4, std::max(1, g.nzc / 2)};
     std::sort(cands.begin(), cands.end());
@@ -828,31 +841,70 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
         f->cur ^= 1;
       }
       (void)hipEventRecord(ctx->ev1, ctx->stream);
-      if (hipEventSynchronize(ctx->ev1) != hipSuccess || hipGetLastError() != hipSuccess) {
-        rc = fail(D3H_ERR_HIP, "autotune launch of tile %d failed", i);
-        break;
-      }
+      if (hipEventSynchronize(ctx->ev1) != hipSuccess || hipGetLastError() != hipSuccess)
+        return fail(D3H_ERR_HIP, "autotune launch of tile %d failed", i);
       float ms = 0;
       (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
       const double per_step = ms / reps / T;
-      if (per_step < best) {
-        best = per_step;
-        bi = i;
-        bz = g2.nzc;
+      if (per_step < *best) {
+        *best = per_step;
+        *bi = i;
+        *bz = g2.nzc;
       }
+    }
+  }
+  return D3H_OK;
+}
+
+int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int temporal_depth, int arith,
+                 int *best_tile, int *best_zchunks, double *best_ms_per_step) {
+  if (!ctx) return fail(D3H_ERR_INVALID, "ctx is NULL");
+  const int D = temporal_depth;
+  if (D != 0 && D != 1 && D != 2 && D != 4 && D != 8)
+    return fail(D3H_ERR_INVALID, "temporal depth %d (0 = choose, 1, 2, 4 or 8)", D);
+  const int fma = arith == D3H_ARITH_FMA ? 1 : 0;
+  d3h_field *f = nullptr;
+  D3H_TRY(d3h_field_create(ctx, nx, ny, nz, dtype, &f));
+  // nz == 1 fields are tuned for the 2D operator (ct = cb = 0), which also
+  // admits the row-streaming tiles
+  const bool rows = nz == 1;
+  const double coef[7] = {0.1, 0.1, 0.1, 0.1, rows ? 0.0 : 0.1, rows ? 0.0 : 0.1, rows ? 0.6 : 0.4};
+  const auto &reg = registry();
+  std::vector<int> depths;
+  if (D == 0) depths = {1, 2, 4, 8};
+  else depths = {D};
+  double best_all = 1e300;
+  int bi_all = -1, bz_all = 0;
+  int rc = D3H_OK;
+  for (int T : depths) {
+    double best = 1e300;
+    int bi = -1, bz = 0;
+    rc = tune_depth(ctx, f, T, fma, coef, &bi, &bz, &best);
+    if (rc != D3H_OK) break;
+    if (bi < 0) continue;
+    const KernelEntry &e = reg[bi];
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] = TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz, best, T};
+    }
+    if (best < best_all) {
+      best_all = best;
+      bi_all = bi;
+      bz_all = bz;
     }
   }
   d3h_field_destroy(f);
   if (rc != D3H_OK) return rc;
-  if (bi < 0) return fail(D3H_ERR_UNSUPPORTED, "no kernel for dtype %d depth %d", dtype, T);
-  {
+  if (bi_all < 0) return fail(D3H_ERR_UNSUPPORTED, "no kernel for dtype %d depth %d", dtype, D);
+  if (D == 0) {
+    const KernelEntry &e = reg[bi_all];
     std::lock_guard<std::mutex> lk(ctx->mu);
-    const KernelEntry &e = reg[bi];
-    ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] = TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz, best};
+    ctx->tuned[tune_key(dtype, 0, fma, nx, ny, nz)] =
+        TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz_all, best_all, e.T};
   }
-  if (best_tile) *best_tile = bi;
-  if (best_zchunks) *best_zchunks = bz;
-  if (best_ms_per_step) *best_ms_per_step = best;
+  if (best_tile) *best_tile = bi_all;
+  if (best_zchunks) *best_zchunks = bz_all;
+  if (best_ms_per_step) *best_ms_per_step = best_all;
   return D3H_OK;
 }
 
@@ -861,14 +913,14 @@ int d3h_autotune_save(d3h_context *ctx, const char *path) {
   std::ofstream os(path);
   if (!os) return fail(D3H_ERR_INVALID, "cannot write %s", path);
   os << "# dtype depth fma log2nx log2ny log2nz vec groups rows_per_wave waves prefetch zchunks "
-        "ms_per_step\n";
+        "ms_per_step entry_depth   (depth 0 = the depth chosen for the size class)\n";
   std::lock_guard<std::mutex> lk(ctx->mu);
   for (const auto &kv : ctx->tuned) {
     const auto &k = kv.first;
     const auto &v = kv.second;
     os << k.dtype << ' ' << k.T << ' ' << k.fma << ' ' << k.cx << ' ' << k.cy << ' ' << k.cz << ' '
        << v.vec << ' ' << v.G << ' ' << v.RPW << ' ' << v.NW << ' ' << v.pf << ' ' << v.zchunks << ' '
-       << v.ms_per_step << '\n';
+       << v.ms_per_step << ' ' << v.T << '\n';
   }
   return os ? D3H_OK : fail(D3H_ERR_INVALID, "write to %s failed", path);
 }
@@ -887,7 +939,12 @@ int d3h_autotune_load(d3h_context *ctx, const char *path) {
     if (!(ss >> k.dtype >> k.T >> k.fma >> k.cx >> k.cy >> k.cz >> v.vec >> v.G >> v.RPW >> v.NW >>
           v.pf >> v.zchunks >> v.ms_per_step))
       return fail(D3H_ERR_INVALID, "malformed line in %s: %s", path, line.c_str());
-    if (find_entry(k.dtype, k.T, k.fma, v.vec, v.G, v.RPW, v.NW, v.pf)) ctx->tuned[k] = v;
+    // entry depth (absent in tables without depth-0 lines: the key's depth)
+    if (!(ss >> v.T)) v.T = k.T;
+    if (v.T <= 0 || (k.T != 0 && v.T != k.T))
+      return fail(D3H_ERR_INVALID, "bad entry depth in %s: %s", path, line.c_str());
+    // entries no longer compiled are skipped
+    if (find_entry(k.dtype, v.T, k.fma, v.vec, v.G, v.RPW, v.NW, v.pf)) ctx->tuned[k] = v;
   }
   return D3H_OK;
 }

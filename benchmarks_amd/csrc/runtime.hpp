@@ -57,9 +57,13 @@ struct TuneVal {
   int vec, G, RPW, NW, pf;  // entry identity (stable across builds)
   int zchunks;
   double ms_per_step;
+  int T;                    // depth of the entry; under a depth-0 key, the tuned depth
 };
-// Size class of a grid: ceil(log2) of each dimension.
+// Size class of a grid: ceil(log2) of each dimension.  Depth 0 keys hold the
+// depth the autotuner chose for the class (d3h_autotune with depth 0).
 TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz);
+// The tuned depth of a size class, or `dflt` when the class has none.
+int tuned_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, int dflt);
 
 // Host <-> pitched device field.  One dense 1-D copy when the dtypes match and
 // rows are unpadded; otherwise a dense 1-D copy through `staging` (at least

@@ -336,9 +336,9 @@ int d3h_server_create(d3h_context *ctx, const d3h_server_config *cfg_in, d3h_ser
   if (cfg_in) cfg = *cfg_in;
   if (cfg.arith != D3H_ARITH_EXACT && cfg.arith != D3H_ARITH_FMA) return fail(D3H_ERR_INVALID, "arith");
   if (cfg.max_active <= 0) cfg.max_active = 4096;
-  if (cfg.temporal_depth <= 0) cfg.temporal_depth = 2;
-  if (cfg.temporal_depth != 1 && cfg.temporal_depth != 2 && cfg.temporal_depth != 4)
-    return fail(D3H_ERR_INVALID, "server depth %d (1, 2 or 4)", cfg.temporal_depth);
+  if (cfg.temporal_depth < 0 || cfg.temporal_depth == 3 || cfg.temporal_depth > 4)
+    return fail(D3H_ERR_INVALID, "server depth %d (0 = tuned per size class, 1, 2 or 4)",
+                cfg.temporal_depth);
   if (cfg.sweep_steps <= 0) cfg.sweep_steps = 8;
   D3H_HIP(hipSetDevice(ctx->device));
   size_t fr = 0, tot = 0;

@@ -85,7 +85,9 @@ int d3h_host_free(void *ptr);
 typedef struct d3h_field d3h_field;
 
 typedef struct {
-  int temporal_depth;  /* steps fused per launch: 1 (plain), 2, 4 or 8       */
+  int temporal_depth;  /* steps fused per launch: 1 (plain), 2, 4 or 8; 0 =
+                          the depth d3h_autotune(..., 0, ...) chose for the
+                          grid's size class, else 1                           */
   int arith;           /* D3H_ARITH_EXACT or D3H_ARITH_FMA                     */
   int tile;            /* kernel/tile index (d3h_tile_info); -1 = tuned choice */
   int zchunks;         /* z-chunks per launch; 0 = the tuned / default value  */
@@ -152,7 +154,10 @@ int d3h_tile_info(int index, d3h_tile_desc *out);
 
 /* Autotuner: time every matching tile (and z-chunk count) on a scratch field of
  * the given shape, remember the fastest for that size class; the table can be
- * saved to / loaded from a text file.  best_tile / best_zchunks may be NULL. */
+ * saved to / loaded from a text file.  best_tile / best_zchunks may be NULL.
+ * temporal_depth 0 searches depths 1, 2, 4 and 8 and also remembers the
+ * fastest depth for the class, used by runs, batches and servers given depth
+ * 0; best_tile then names the chosen depth's tile (d3h_tile_info). */
 int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype,
                  int temporal_depth, int arith, int *best_tile, int *best_zchunks,
                  double *best_ms_per_step);
@@ -206,8 +211,9 @@ int d3h_batch_destroy(d3h_batch *b);
 /* Upload problem i's initial field (dense host array of host_dtype). */
 int d3h_batch_upload(d3h_batch *b, int i, const void *host, int host_dtype);
 int d3h_batch_download(d3h_batch *b, int i, void *host, int host_dtype);
-/* Run every problem to its step count.  arith as above; temporal_depth 0 =
- * tuned per size class; use_graph 1 = capture + replay. */
+/* Run every problem to its step count.  arith as above; temporal_depth 1, 2,
+ * 4 or 8, or 0 = the depth d3h_autotune(..., 0, ...) chose for each problem's
+ * size class, else 2; use_graph 1 = capture + replay. */
 int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph,
                   d3h_timing *timing);
 
@@ -221,7 +227,8 @@ typedef struct {
   size_t hbm_budget_bytes;  /* device bytes the server may hold; 0 = 90% of free */
   int max_active;           /* max concurrently resident requests; 0 = 4096     */
   int arith;                /* D3H_ARITH_EXACT / D3H_ARITH_FMA                   */
-  int temporal_depth;       /* steps per sweep launch; 0 = 2                     */
+  int temporal_depth;       /* steps per sweep launch: 1, 2 or 4; 0 = the tuned
+                               depth of each request's size class, else 2     */
   int sweep_steps;          /* steps per scheduling round; 0 = 8                 */
 } d3h_server_config;
 

@@ -82,3 +82,43 @@ def test_tb_fma_within_tolerance(ctx):
         out, _ = run(ctx, f0, d3.F32, d3.F32, d3.coefficients(k), 40, depth=depth,
                      arith=d3.ARITH_FMA)
         assert oracle.diff_norms(out, want)[0] <= 2e-6, depth
+
+
+def test_autotuner_chooses_depth(tmp_path):
+    """d3h_autotune with depth 0 searches depths 1/2/4/8 and remembers the
+    fastest per size class; depth-0 runs and batches then use it, bitwise
+    equal to plain steps, also after a save/load round trip."""
+    shape = (160, 96, 72)
+    k = d3.constants_f32(*shape)
+    f0 = d3.add_noise_f32(d3.initialize_f32(*shape, k), 9, 1e-3)
+    coef = d3.coefficients(k)
+    want = oracle.run_f32(f0, k[8:15], 13)
+    path = str(tmp_path / "tuning.txt")
+    with d3.Context(0) as c:
+        t = c.autotune(*shape, d3.F32, 0, d3.ARITH_EXACT)
+        chosen = d3.tiles()[t["tile"]]
+        print("chosen", chosen, t, flush=True)
+        for T in (1, 2, 4, 8):          # every depth's result is remembered too
+            out, tm = run(c, f0, d3.F32, d3.F32, coef, 13, depth=T)
+            assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), T
+        out, tm = run(c, f0, d3.F32, d3.F32, coef, 13, depth=0)
+        assert tm["tile"] == t["tile"] and tm["temporal_depth"] == chosen["temporal_depth"]
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+        c.save_tuning(path)
+        p = {"nx": shape[0], "ny": shape[1], "nz": shape[2], "dtype": d3.F32, "steps": 13,
+             "coef": coef}
+        with d3.Batch(c, [p, p]) as b:
+            b.upload(0, f0, d3.F32)
+            b.upload(1, f0, d3.F32)
+            tb = b.run(depth=0, graph=False)
+            assert np.array_equal(b.download(1, d3.F32).view(np.uint32), want.view(np.uint32))
+    with open(path) as fh:
+        lines = [l.split() for l in fh if not l.startswith("#")]
+    assert any(l[1] == "0" and int(l[13]) == chosen["temporal_depth"] for l in lines), lines
+    with d3.Context(0) as c2:
+        out, tm = run(c2, f0, d3.F32, d3.F32, coef, 13, depth=0)
+        assert tm["temporal_depth"] == 1          # nothing tuned: plain steps
+        c2.load_tuning(path)
+        out, tm = run(c2, f0, d3.F32, d3.F32, coef, 13, depth=0)
+        assert tm["tile"] == t["tile"]
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
