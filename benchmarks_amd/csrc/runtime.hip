@@ -1002,6 +1002,15 @@ static int env_int(const char *name, int dflt) {
   return (v && *v) ? atoi(v) : dflt;
 }
 
+// Knobs of the drop-in path (environment):
+//   D3H_DEVICE          device index (default 0)
+//   D3H_TEMPORAL_DEPTH  1 (default), 2, 4, 8, or 0 = the tuned depth
+//   D3H_ARITH           0 EXACT (default), 1 FMA
+//   D3H_USE_GRAPH       1 = replay a captured hipGraph
+//   D3H_TUNE_FILE       tuning table loaded at initialize_hip when it exists
+//   D3H_AUTOTUNE        1 = tune this grid at initialize_benchmark_hip (for
+//                       D3H_TEMPORAL_DEPTH, or every depth when it is 0) and
+//                       append the result to D3H_TUNE_FILE when set
 int initialize_hip(int argc, char **argv, int nx, int ny, int nz) {
   (void)argc;
   (void)argv;
@@ -1009,6 +1018,11 @@ int initialize_hip(int argc, char **argv, int nx, int ny, int nz) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return fail(D3H_ERR_INVALID, "dims %dx%dx%d", nx, ny, nz);
   D3H_TRY(d3h_context_create(env_int("D3H_DEVICE", 0), &g_drop.ctx));
   g_drop.nx = nx; g_drop.ny = ny; g_drop.nz = nz;
+  const char *tf = getenv("D3H_TUNE_FILE");
+  if (tf && *tf) {
+    std::ifstream probe(tf);
+    if (probe) D3H_TRY(d3h_autotune_load(g_drop.ctx, tf));
+  }
   return D3H_OK;
 }
 
@@ -1017,6 +1031,12 @@ int initialize_benchmark_hip(int nx, int ny, int nz) {
   if (nx != g_drop.nx || ny != g_drop.ny || nz != g_drop.nz)
     return fail(D3H_ERR_INVALID, "dims differ from initialize_hip");
   if (g_drop.field) return fail(D3H_ERR_STATE, "benchmark already initialised");
+  if (env_int("D3H_AUTOTUNE", 0)) {
+    D3H_TRY(d3h_autotune(g_drop.ctx, nx, ny, nz, D3H_F32, env_int("D3H_TEMPORAL_DEPTH", 1),
+                         env_int("D3H_ARITH", D3H_ARITH_EXACT), nullptr, nullptr, nullptr));
+    const char *tf = getenv("D3H_TUNE_FILE");
+    if (tf && *tf) D3H_TRY(d3h_autotune_save(g_drop.ctx, tf));
+  }
   return d3h_field_create(g_drop.ctx, nx, ny, nz, D3H_F32, &g_drop.field);
 }
 

@@ -8,13 +8,15 @@
 // computed by the engine on an nx x ny x 1 device field whose rows are stored
 // in reverse order (d3h_field_set_y_reversed) with ct = cb = 0, which gives
 // the family's bits.  Engine knobs: D3H_TEMPORAL_DEPTH, D3H_ARITH,
-// D3H_USE_GRAPH, D3H_DEVICE, as for the 3D driver.
+// D3H_USE_GRAPH, D3H_DEVICE, D3H_TUNE_FILE, D3H_AUTOTUNE, as for the 3D
+// driver.
 #pragma once
 
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <fstream>
 #include <iostream>
 #include <string>
 
@@ -91,6 +93,15 @@ class Diffusion2DHIP {
     CheckStatus(d3h_context_create(EnvInt("D3H_DEVICE", 0), &ctx_), "d3h_context_create");
     CheckStatus(d3h_field_create(ctx_, nx_, ny_, 1, D3H_F32, &field_), "d3h_field_create");
     CheckStatus(d3h_field_set_y_reversed(field_, 1), "d3h_field_set_y_reversed");
+    const char *tf = getenv("D3H_TUNE_FILE");
+    if (tf && *tf && std::ifstream(tf))
+      CheckStatus(d3h_autotune_load(ctx_, tf), "d3h_autotune_load");
+    if (EnvInt("D3H_AUTOTUNE", 0)) {
+      CheckStatus(d3h_autotune(ctx_, nx_, ny_, 1, D3H_F32, EnvInt("D3H_TEMPORAL_DEPTH", 1),
+                               EnvInt("D3H_ARITH", D3H_ARITH_EXACT), nullptr, nullptr, nullptr),
+                  "d3h_autotune");
+      if (tf && *tf) CheckStatus(d3h_autotune_save(ctx_, tf), "d3h_autotune_save");
+    }
   }
   void InitializeInput() {
     CheckStatus(d3h_initialize2d_f32(f1_, nx_, ny_, constants_, 0.0f), "d3h_initialize2d_f32");

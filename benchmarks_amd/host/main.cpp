@@ -4,8 +4,8 @@
 // getopt loop), plus --warmup and --nd N (diffusion/main.cc:86-129; --nd 2
 // runs the diffusion/ family's 2D benchmark) and --nx/--ny/--nz for
 // non-cubic grids.  Engine knobs come from the environment:
-// D3H_TEMPORAL_DEPTH (1, 2, 4, 8), D3H_ARITH (0 exact, 1 fma),
-// D3H_USE_GRAPH (0/1), D3H_DEVICE.
+// D3H_TEMPORAL_DEPTH (1, 2, 4, 8; 0 = tuned), D3H_ARITH (0 exact, 1 fma),
+// D3H_USE_GRAPH (0/1), D3H_DEVICE, D3H_TUNE_FILE, D3H_AUTOTUNE.
 #include <getopt.h>
 #include <stdlib.h>
 

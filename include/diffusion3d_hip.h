@@ -50,7 +50,12 @@ enum { D3H_ARITH_EXACT = 0, D3H_ARITH_FMA = 1 };
  * kernel-only timer for DisplayResult's "Kernel-only performance" block
  * (diffusion3d/diffusion3d_cuda.cu:373-384).  run_kernel_hip advances f1_host
  * by exactly `count` steps (H2D, launches, D2H; synchronous), like
- * Diffusion3DCUDA::RunKernel (diffusion3d/diffusion3d_cuda.cu:353-371). */
+ * Diffusion3DCUDA::RunKernel (diffusion3d/diffusion3d_cuda.cu:353-371).
+ * Environment: D3H_DEVICE, D3H_TEMPORAL_DEPTH (1, 2, 4, 8; 0 = tuned),
+ * D3H_ARITH (0 EXACT, 1 FMA), D3H_USE_GRAPH (1 = hipGraph replay),
+ * D3H_TUNE_FILE (tuning table loaded by initialize_hip when it exists) and
+ * D3H_AUTOTUNE (1 = tune the grid in initialize_benchmark_hip and save the
+ * table to D3H_TUNE_FILE when set). */
 int initialize_hip(int argc, char **argv, int nx, int ny, int nz);
 int initialize_benchmark_hip(int nx, int ny, int nz);
 int run_kernel_hip(int count, float *f1_host, int nx, int ny, int nz,

@@ -218,3 +218,22 @@ def test_driver_matches_reference_output_format():
     assert "Running diffusion3d/hip" in out and "Kernel-only performance:" in out
     acc = re.search(r"Accuracy\s+:\s+(\S+)", out).group(1)
     assert acc == "7.227031e-07", out
+
+
+def test_driver_autotune_knobs(tmp_path):
+    """D3H_AUTOTUNE / D3H_TUNE_FILE / D3H_TEMPORAL_DEPTH=0 on the drop-in path:
+    the tuned run prints the reference's Accuracy, writes a table with the
+    chosen depth, and a second run reuses the table."""
+    table = tmp_path / "tune.txt"
+    env = dict(os.environ, D3H_AUTOTUNE="1", D3H_TEMPORAL_DEPTH="0", D3H_TUNE_FILE=str(table))
+    r = subprocess.run([N.DRIVER_PATH, "--size", "128", "--count", "100"], capture_output=True,
+                       text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr
+    assert re.search(r"Accuracy\s+:\s+(\S+)", r.stdout).group(1) == "7.227031e-07", r.stdout
+    rows = [l.split() for l in table.read_text().splitlines() if not l.startswith("#")]
+    assert any(l[1] == "0" and l[3:6] == ["7", "7", "7"] for l in rows), rows
+    env.pop("D3H_AUTOTUNE")
+    r = subprocess.run([N.DRIVER_PATH, "--size", "128", "--count", "100"], capture_output=True,
+                       text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr
+    assert re.search(r"Accuracy\s+:\s+(\S+)", r.stdout).group(1) == "7.227031e-07", r.stdout
