@@ -24,15 +24,17 @@ D3H_ERR_UNSTABLE = 7
 
 F32, F64, BF16, F16 = 0, 1, 2, 3
 ARITH_EXACT, ARITH_FMA = 0, 1
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 
 c_int, c_double, c_float, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_void_p
 c_size_t, c_int64, c_uint64, c_char_p = ctypes.c_size_t, ctypes.c_int64, ctypes.c_uint64, ctypes.c_char_p
+c_uint = ctypes.c_uint
 P = ctypes.POINTER
 
 
 class RunConfig(ctypes.Structure):
     _fields_ = [("temporal_depth", c_int), ("arith", c_int), ("tile", c_int),
-                ("zchunks", c_int), ("use_graph", c_int)]
+                ("zchunks", c_int), ("use_graph", c_int), ("rounding", c_int), ("seed", c_uint)]
 
 
 class Timing(ctypes.Structure):
@@ -52,7 +54,7 @@ class TileDesc(ctypes.Structure):
 
 class Problem(ctypes.Structure):
     _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("dtype", c_int), ("steps", c_int),
-                ("coef", c_double * 7)]
+                ("coef", c_double * 7), ("rounding", c_int), ("seed", c_uint)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -27,6 +27,10 @@ int check_problem(const d3h_problem &p) {
     sum += p.coef[i];
   }
   if (std::fabs(sum - 1.0) > 1e-5) return fail(D3H_ERR_UNSTABLE, "coefficients sum to %.9g, not 1", sum);
+  if (p.rounding != D3H_ROUND_NEAREST && p.rounding != D3H_ROUND_STOCHASTIC)
+    return fail(D3H_ERR_INVALID, "rounding %d", p.rounding);
+  if (p.rounding == D3H_ROUND_STOCHASTIC && p.dtype != D3H_BF16)
+    return fail(D3H_ERR_UNSUPPORTED, "stochastic rounding is for BF16 storage (dtype %d)", p.dtype);
   return D3H_OK;
 }
 
@@ -67,7 +71,7 @@ static int batch_zchunks(int nz, int T) { return std::max(1, nz / std::max(32, 8
 int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int fma, SweepPlan *plan) {
   const int n = (int)ms.size();
   std::vector<const KernelEntry *> ea(n, nullptr), eb(n, nullptr);
-  std::vector<int> qa(n, 0), rb(n, 0);
+  std::vector<int> qa(n, 0), rb(n, 0), ta(n, 1);
   for (int i = 0; i < n; ++i) {
     const PlanMember &m = ms[i];
     // depth 0: the depth the autotuner chose for the member's size class, else 2
@@ -79,6 +83,7 @@ int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int f
     }
     if (!e) return fail(D3H_ERR_UNSUPPORTED, "no batched kernel for dtype %d", m.dtype);
     ea[i] = e;
+    ta[i] = Tm;
     qa[i] = m.steps / Tm;
     rb[i] = m.steps % Tm;
     if (rb[i]) {
@@ -111,6 +116,10 @@ int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int f
         for (int k = 0; k < 7; ++k) m.coef[k] = pm.coef[k];
         m.block_start = blocks;
         m.parity0 = tail ? (pm.parity0 + qa[i]) & 1 : pm.parity0;
+        // the kernel numbers the steps of launch k from step0 + k * T
+        m.geo.sr = pm.sr;
+        m.geo.seed = pm.seed;
+        m.geo.step0 = pm.step_base + (tail ? qa[i] * ta[i] : 0);
         c.table.push_back(m);
         c.launches.push_back(cnt[i]);
         for (unsigned k = 0; k < nb; ++k) c.block_member.push_back((int)c.table.size() - 1);
@@ -347,6 +356,9 @@ int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph, d3
       m.parity0 = b->cur[i];
       m.steps = p.steps;
       for (int k = 0; k < 7; ++k) m.coef[k] = p.coef[k];
+      m.sr = p.rounding == D3H_ROUND_STOCHASTIC;
+      m.seed = p.seed;
+      m.step_base = 0;  // step numbers start at 0 in every run
       ms.push_back(m);
     }
     SweepPlan plan;

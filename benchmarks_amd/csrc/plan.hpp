@@ -21,6 +21,9 @@ struct PlanMember {
   int parity0;   // buffer holding the input
   int steps;     // steps to advance in this sweep
   double coef[7];
+  int sr;        // stochastic rounding of BF16 storage
+  unsigned seed;
+  int step_base; // steps the member has taken before this sweep
 };
 
 struct ClassPlan {
@@ -54,7 +57,8 @@ void plan_fill(const SweepPlan &plan, char *host);
 int plan_enqueue(std::vector<ClassPlan> &cs, void *dev_tables, hipStream_t main,
                  const std::vector<hipStream_t> &streams, const std::vector<hipEvent_t> &events,
                  int *launches);
-// Validate a problem: dims, dtype, steps >= 0, coefficients >= 0 summing to 1.
+// Validate a problem: dims, dtype, steps >= 0, coefficients >= 0 summing to 1,
+// rounding (stochastic only with BF16 storage).
 int check_problem(const d3h_problem &p);
 
 }  // namespace d3h

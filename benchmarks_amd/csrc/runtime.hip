@@ -245,7 +245,7 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
 }
 
 int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEntry &e, int zchunks,
-                  hipStream_t stream, int *launches, unsigned *blocks_out) {
+                  int sr, unsigned seed, hipStream_t stream, int *launches, unsigned *blocks_out) {
   const int T = e.T;
   const int nfull = count / T, rem = count % T;
   Geometry g{}, g1{};
@@ -259,12 +259,19 @@ int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEnt
     if (!plain) return fail(D3H_ERR_UNSUPPORTED, "no plain kernel for dtype %d", f->dtype);
     D3H_TRY(field_geometry(*plain, f, 0, &g1, &nb1));
   }
+  g.sr = g1.sr = sr;
+  g.seed = g1.seed = seed;
+  // step numbers matter only under stochastic rounding; keeping them 0
+  // otherwise leaves captured graphs reusable across calls
+  const long long base = sr ? f->steps_done : 0;
   int n = 0;
   for (int i = 0; i < nfull; ++i, ++n) {
+    g.step0 = (int)(base + (long long)i * T);
     launch_entry(e, f, f->buf[f->cur], f->buf[f->cur ^ 1], g, coef, nb, stream);
     f->cur ^= 1;
   }
   for (int i = 0; i < rem; ++i, ++n) {
+    g1.step0 = (int)(base + (long long)nfull * T + i);
     launch_entry(*plain, f, f->buf[f->cur], f->buf[f->cur ^ 1], g1, coef, nb1, stream);
     f->cur ^= 1;
   }
@@ -473,6 +480,8 @@ void d3h_run_config_default(d3h_run_config *cfg) {
   cfg->tile = -1;
   cfg->zchunks = 0;
   cfg->use_graph = 0;
+  cfg->rounding = D3H_ROUND_NEAREST;
+  cfg->seed = 0;
 }
 
 int d3h_context_create(int device, d3h_context **out) {
@@ -626,6 +635,7 @@ int d3h_field_upload(d3h_field *f, const void *host, int host_dtype) {
   D3H_TRY(copy_in(host, host_dtype, f->buf[f->cur], f->dtype, f->pitch, f->nx, f->ny, f->nz, f->staging,
                   ctx->stream, f->yrev));
   D3H_HIP(hipStreamSynchronize(ctx->stream));
+  f->steps_done = 0;
   return D3H_OK;
 }
 
@@ -655,13 +665,21 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
   D3H_HIP(hipSetDevice(ctx->device));
   const KernelEntry *e = nullptr;
   int zchunks = 0;
+  if (cfg.rounding != D3H_ROUND_NEAREST && cfg.rounding != D3H_ROUND_STOCHASTIC)
+    return fail(D3H_ERR_INVALID, "rounding %d", cfg.rounding);
+  const int sr = cfg.rounding == D3H_ROUND_STOCHASTIC;
+  if (sr && f->dtype != D3H_BF16)
+    return fail(D3H_ERR_UNSUPPORTED, "stochastic rounding is for BF16 storage (field dtype %d)", f->dtype);
+  if (sr && f->steps_done + count > INT_MAX)
+    return fail(D3H_ERR_INVALID, "step numbers of stochastic rounding exceed 2^31");
   D3H_TRY(resolve_entry(ctx, f->nx, f->ny, f->nz, f->dtype, cfg, rows_ok(f, coef), &e, &zchunks));
   int launches = 0;
   unsigned blocks = 0;
   float ms = 0.f;
   if (count > 0) {
     if (cfg.use_graph) {
-      std::vector<long long> key = {count, (long long)(intptr_t)e, zchunks, f->cur};
+      std::vector<long long> key = {count, (long long)(intptr_t)e, zchunks, f->cur, sr,
+                                    sr ? (long long)cfg.seed : 0, sr ? f->steps_done : 0};
       for (int i = 0; i < 7; ++i) {
         long long bits;
         memcpy(&bits, &coef[i], 8);
@@ -672,7 +690,8 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
         const int cur0 = f->cur;
         hipGraph_t graph = nullptr;
         D3H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_steps(f, count, coef, *e, zchunks, ctx->stream, &launches, &blocks);
+        int rc = enqueue_steps(f, count, coef, *e, zchunks, sr, cfg.seed, ctx->stream, &launches,
+                               &blocks);
         hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
         f->cur = cur0;
         if (rc != D3H_OK) {
@@ -695,11 +714,12 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
       if (launches & 1) f->cur ^= 1;
     } else {
       D3H_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-      D3H_TRY(enqueue_steps(f, count, coef, *e, zchunks, ctx->stream, &launches, &blocks));
+      D3H_TRY(enqueue_steps(f, count, coef, *e, zchunks, sr, cfg.seed, ctx->stream, &launches, &blocks));
       D3H_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     }
     D3H_HIP(hipEventSynchronize(ctx->ev1));
     D3H_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    f->steps_done += count;
   }
   if (timing) {
     timing->kernel_ms = ms;

@@ -101,6 +101,7 @@ struct d3h_field {
   void *staging = nullptr;
   size_t staging_bytes = 0;
   bool yrev = false;  // rows stored in reverse order (see d3h_field_set_y_reversed)
+  long long steps_done = 0;  // steps since the last upload (stochastic rounding)
   // cached captured launch sequence
   hipGraphExec_t gexec = nullptr;
   std::vector<long long> gkey;
@@ -119,8 +120,10 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
 int field_geometry(const KernelEntry &e, const d3h_field *f, int zchunks, Geometry *g,
                    unsigned *blocks);
 // Enqueue `count` steps from f->buf[f->cur] on `stream` (no sync); updates cur.
+// sr / seed: stochastic rounding of BF16 storage, step numbers continuing from
+// f->steps_done (the caller advances steps_done).
 int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEntry &e, int zchunks,
-                  hipStream_t stream, int *launches, unsigned *blocks);
+                  int sr, unsigned seed, hipStream_t stream, int *launches, unsigned *blocks);
 // Convert nx*ny*nz cells between device arrays of two dtypes; a row pitch of
 // 0 means dense (pitch = nx).  Narrowing rounds to nearest even (from f64 to
 // bf16/f16 it rounds to f32 first).

@@ -251,6 +251,9 @@ int round_once(d3h_server *s, bool *did_work) {
       m.parity0 = r->cur;
       m.steps = std::min(r->remaining, s->cfg.sweep_steps);
       for (int k = 0; k < 7; ++k) m.coef[k] = r->p.coef[k];
+      m.sr = r->p.rounding == D3H_ROUND_STOCHASTIC;
+      m.seed = r->p.seed;
+      m.step_base = r->p.steps - r->remaining;  // the request's own step numbers
       ms.push_back(m);
     }
     SweepPlan plan;

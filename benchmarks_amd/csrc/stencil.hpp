@@ -78,6 +78,12 @@ struct Geometry {
   long long plane;
   int ntx, nty, nzc;
   int ox, oy, oz, hx;
+  // Stochastic rounding of BF16 storage (sr = 1): every stored time level is
+  // rounded with a hash of (seed, cell, step); step0 is the number of steps
+  // the field had taken before this launch (a batched launch adds launch * T).
+  int sr;
+  unsigned seed;
+  int step0;
 };
 
 template <typename C> struct Coef { C ce, cw, cn, cs, ct, cb, cc; };
@@ -169,6 +175,28 @@ struct Pack {
     }
   }
 };
+
+// Deterministic stochastic rounding FP32 -> BF16.  A 32-bit hash of (seed,
+// linear cell index, step) supplies 16 random bits that are added below the
+// BF16 mantissa before truncation, so x rounds away from zero with
+// probability equal to its distance from the lower-magnitude neighbour (in
+// units of that ulp): unbiased, and reproducible on any device or on the
+// host.  The hash is the "lowbias32" integer mixer (C. Wellons, hash-prospector)
+// applied to a Weyl-style combination of the three inputs.
+__device__ __forceinline__ uint32_t sr_hash(uint32_t seed, uint32_t cell, uint32_t step) {
+  uint32_t x = cell * 0x9E3779B1u + step * 0x85EBCA77u + seed;
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ float sr_bf16(float x, uint32_t h) {
+  const uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return __uint_as_float(to_bf16(x) << 16);  // inf / nan
+  return __uint_as_float((u + (h & 0xffffu)) & 0xffff0000u);
+}
 
 // Round a compute value to storage precision and back (identity for f32/f64).
 template <int S>
@@ -264,6 +292,7 @@ struct TileRunner {
   C *xch;
   int lane, wave;
   int xlane;            // x of this lane's cell 0 in group 0
+  int yrow0;            // y of this wave's row 0
   int zl0, zl1, zo0, zo1;
   bool edge_xy;         // tile holds the last column/row strictly inside it
   int plane_bytes;
@@ -289,6 +318,7 @@ struct TileRunner {
     zl1 = min(zo1 + T, geo.nz);
     edge_xy = (xl0 + TILE_X > geo.nx) || (yl0 + TILE_Y > geo.ny);
     xlane = xl0 + lane * VEC;
+    yrow0 = yl0 + wave * RPW;
     // a negative plane stride walks the planes downwards (row streaming)
     plane_bytes = (int)((geo.plane < 0 ? -geo.plane : geo.plane) * (long long)sizeof(ST));
 #pragma unroll
@@ -428,6 +458,22 @@ struct TileRunner {
         }
       }
     }
+    if constexpr (S == kBF16) {
+      if (geo.sr) {
+        const uint32_t step = (uint32_t)(geo.step0 + s - 1);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+          const uint32_t row = ((uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r)) * (uint32_t)geo.nx;
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+              const uint32_t cell = row + (uint32_t)(xlane + g * GSTRIDE + v);
+              out[r][g * VEC + v] = sr_bf16(out[r][g * VEC + v], sr_hash(geo.seed, cell, step));
+            }
+        }
+      }
+    }
     if constexpr (s < T) {
 #pragma unroll
       for (int r = 0; r < RPW; ++r)
@@ -535,13 +581,15 @@ stencil_batch_kernel(const BatchMember *__restrict__ members, const int *__restr
   const Geometry geo = d.geo;
   const int local = lb - d.block_start;
   const int par = (d.parity0 + launch) & 1;
+  Geometry geo_l = geo;
+  geo_l.step0 += launch * T;
   const Coef<C> k{(C)d.coef[0], (C)d.coef[1], (C)d.coef[2], (C)d.coef[3],
                   (C)d.coef[4], (C)d.coef[5], (C)d.coef[6]};
   const int tx = local % geo.ntx;
   const int rest = local / geo.ntx;
   const int ty = rest % geo.nty;
   const int tz = rest / geo.nty;
-  R tr(static_cast<const ST *>(d.buf[par]), static_cast<ST *>(d.buf[par ^ 1]), geo, k, xch, tx, ty,
+  R tr(static_cast<const ST *>(d.buf[par]), static_cast<ST *>(d.buf[par ^ 1]), geo_l, k, xch, tx, ty,
        tz);
   tr.run();
 }

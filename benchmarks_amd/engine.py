@@ -201,8 +201,12 @@ class Field:
         check(lib().d3h_field_download(self.handle, _dptr(out), host_dtype), "d3h_field_download")
         return out
 
-    def run(self, count, coef, depth=1, arith=N.ARITH_EXACT, tile=-1, zchunks=0, graph=False):
-        cfg = N.RunConfig(depth, arith, tile, zchunks, 1 if graph else 0)
+    def run(self, count, coef, depth=1, arith=N.ARITH_EXACT, tile=-1, zchunks=0, graph=False,
+            rounding=N.ROUND_NEAREST, seed=0):
+        """Advance `count` steps.  rounding=ROUND_STOCHASTIC (BF16 fields) rounds
+        every stored level stochastically with `seed`; step numbers continue
+        from the last upload."""
+        cfg = N.RunConfig(depth, arith, tile, zchunks, 1 if graph else 0, rounding, seed)
         t = N.Timing()
         check(lib().d3h_field_run(self.handle, count, _coef_array(coef), ctypes.byref(cfg),
                                   ctypes.byref(t)), "d3h_field_run")
@@ -258,7 +262,8 @@ def run_kernel(count, f1, coef):
 class Batch:
     """Many independent grids advanced together (d3h_batch_*).
 
-    problems: list of dicts with nx, ny, nz, dtype, steps, coef (7 values)."""
+    problems: list of dicts with nx, ny, nz, dtype, steps, coef (7 values) and
+    optionally rounding (ROUND_NEAREST / ROUND_STOCHASTIC) and seed."""
 
     def __init__(self, ctx, problems):
         arr = (N.Problem * len(problems))()
@@ -267,6 +272,8 @@ class Batch:
             arr[i].dtype, arr[i].steps = p.get("dtype", N.F32), p["steps"]
             for j, c in enumerate(p["coef"]):
                 arr[i].coef[j] = float(c)
+            arr[i].rounding = p.get("rounding", N.ROUND_NEAREST)
+            arr[i].seed = p.get("seed", 0)
         h = ctypes.c_void_p()
         check(lib().d3h_batch_create(ctx.handle, arr, len(problems), ctypes.byref(h)),
               "d3h_batch_create")
@@ -361,6 +368,8 @@ class Server:
         p.dtype, p.steps = problem.get("dtype", N.F32), problem["steps"]
         for j, c in enumerate(problem["coef"]):
             p.coef[j] = float(c)
+        p.rounding = problem.get("rounding", N.ROUND_NEAREST)
+        p.seed = problem.get("seed", 0)
         out_dtype = {np.dtype(np.float32): N.F32, np.dtype(np.float64): N.F64,
                      np.dtype(np.float16): N.F16, np.dtype(np.uint16): N.BF16}[out.dtype]
         rid = ctypes.c_int64()

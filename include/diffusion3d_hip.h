@@ -44,6 +44,15 @@ enum { D3H_F32 = 0, D3H_F64 = 1, D3H_BF16 = 2, D3H_F16 = 3 };
  * way a contracting compiler does it, fma(cc,c,cw*w) then a chain of fmas. */
 enum { D3H_ARITH_EXACT = 0, D3H_ARITH_FMA = 1 };
 
+/* Rounding of BF16 storage.  NEAREST: round to nearest even after every step.
+ * STOCHASTIC: round up or down with probability given by the distance to the
+ * two neighbours, using a deterministic hash of (seed, linear cell index,
+ * step number) -- unbiased, so the small per-step changes of this problem are
+ * not lost to rounding, and reproducible bitwise.  Step numbers count from the
+ * field's last upload, from 0 in every d3h_batch_run, and from a server
+ * request's first step. */
+enum { D3H_ROUND_NEAREST = 0, D3H_ROUND_STOCHASTIC = 1 };
+
 /* ---- drop-in variant ABI ---------------------------------------------------
  * The four Physis entry points of the reference
  * (diffusion3d/diffusion3d_physis.h:7-17), same argument meaning, plus a
@@ -97,6 +106,8 @@ typedef struct {
   int tile;            /* kernel/tile index (d3h_tile_info); -1 = tuned choice */
   int zchunks;         /* z-chunks per launch; 0 = the tuned / default value  */
   int use_graph;       /* 1: capture the launch sequence in a hipGraph         */
+  int rounding;        /* D3H_ROUND_*; STOCHASTIC needs a BF16 field           */
+  unsigned int seed;   /* seed of stochastic rounding                          */
 } d3h_run_config;
 
 typedef struct {
@@ -208,6 +219,8 @@ typedef struct {
   int dtype;        /* D3H_F32 / D3H_BF16 / D3H_F16 / D3H_F64 */
   int steps;        /* exact number of steps for this problem */
   double coef[7];   /* ce cw cn cs ct cb cc                    */
+  int rounding;     /* D3H_ROUND_*; STOCHASTIC needs BF16      */
+  unsigned int seed;
 } d3h_problem;
 
 int d3h_batch_create(d3h_context *ctx, const d3h_problem *problems, int count,

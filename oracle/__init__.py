@@ -43,6 +43,9 @@ def lib():
             "oracle_run_lp": [_u16, _u16, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int,
                               ctypes.c_int],
             "oracle_f32_to_lp": [_f, _u16, ctypes.c_long, ctypes.c_int],
+            "oracle_run_bf16_sr": [_u16, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int,
+                                   ctypes.c_uint32, ctypes.c_int],
+            "oracle_sr_hash": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32],
             "oracle_lp_to_f32": [_u16, _f, ctypes.c_long, ctypes.c_int],
             "oracle_accuracy_f32": [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int],
             "oracle_rms_vs_analytical_f32": [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f,
@@ -57,7 +60,7 @@ def lib():
             "oracle_diff_norms_f32": [_f, _f, ctypes.c_long, _d],
             "oracle_diff_norms_f64": [_d, _d, ctypes.c_long, _d],
         }
-        res = {"oracle_accuracy_f32": ctypes.c_float, "oracle_accuracy2d_f32": ctypes.c_float, "oracle_rms_vs_analytical_f32": ctypes.c_double,
+        res = {"oracle_sr_hash": ctypes.c_uint32, "oracle_accuracy_f32": ctypes.c_float, "oracle_accuracy2d_f32": ctypes.c_float, "oracle_rms_vs_analytical_f32": ctypes.c_double,
                "oracle_rms_vs_analytical_f64": ctypes.c_double}
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -147,6 +150,19 @@ def run_lp(bits, coef, count, kind):
     nz, ny, nx = a.shape
     lib().oracle_run_lp(_p(a, _u16), _p(b, _u16), nx, ny, nz, _p(c, _f), count, kind)
     return a
+
+
+def run_bf16_sr(bits, coef, count, seed, step0=0):
+    """`count` steps of BF16 storage with stochastic rounding (new array)."""
+    a = np.array(bits, dtype=np.uint16, copy=True, order="C")
+    c = np.ascontiguousarray(coef, np.float32)
+    nz, ny, nx = a.shape
+    lib().oracle_run_bf16_sr(_p(a, _u16), nx, ny, nz, _p(c, _f), count, seed, step0)
+    return a
+
+
+def sr_hash(seed, cell, step):
+    return int(lib().oracle_sr_hash(seed, cell, step))
 
 
 def accuracy_f32(f, k, count):

@@ -258,6 +258,45 @@ void oracle_run_lp(uint16_t *f1, uint16_t *f2, int nx, int ny, int nz,
   free(b);
 }
 
+/* Stochastic rounding to BF16 (no reference counterpart).  The hash is
+ * "lowbias32" (C. Wellons, hash-prospector) of a Weyl-style combination of the
+ * seed, the linear cell index and the step number. */
+uint32_t oracle_sr_hash(uint32_t seed, uint32_t cell, uint32_t step) {
+  uint32_t x = cell * 0x9E3779B1u + step * 0x85EBCA77u + seed;
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+void oracle_run_bf16_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF],
+                        int count, uint32_t seed, int step0) {
+  const long n = (long)nx * ny * nz;
+  float *a = (float *)malloc(sizeof(float) * (size_t)n);
+  float *b = (float *)malloc(sizeof(float) * (size_t)n);
+  oracle_lp_to_f32(f1, a, n, 0);
+  for (int i = 0; i < count; ++i) {
+    oracle_step_f32(a, b, nx, ny, nz, c);
+    const uint32_t step = (uint32_t)(step0 + i);
+    for (long j = 0; j < n; ++j) {
+      uint32_t u;
+      memcpy(&u, &b[j], 4);
+      uint16_t h;
+      if ((u & 0x7f800000u) == 0x7f800000u) {
+        h = to_bf16(b[j]);
+      } else {
+        h = (uint16_t)((u + (oracle_sr_hash(seed, (uint32_t)j, step) & 0xffffu)) >> 16);
+      }
+      a[j] = from_bf16(h);
+    }
+  }
+  oracle_f32_to_lp(a, f1, n, 0);
+  free(a);
+  free(b);
+}
+
 /* ---------------------------------------------------------------------------
  * a7: Baseline::GetAccuracy + GetCorrectAnswer (diffusion3d/baseline.cc:51-60,
  * diffusion3d/diffusion3d.h:113-120).  time = count * dt_ in float.

@@ -265,3 +265,91 @@ def test_noise_properties():
     assert np.array_equal(a, b) and not np.array_equal(a, c)
     assert a.min() >= -1e-3 and a.max() < 1e-3
     assert abs(a.mean()) < 1e-5 and abs(a.std() - 1e-3 / np.sqrt(3)) < 1e-5
+
+
+# ---- stochastic rounding of BF16 storage (no reference counterpart) --------
+
+def _lowbias32_np(x):
+    x = np.asarray(x, dtype=np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7feb352d) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846ca68b) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def _sr_hash_np(seed, cell, step):
+    cell = np.asarray(cell, dtype=np.uint64)
+    return _lowbias32_np((cell * 0x9E3779B1 + step * 0x85EBCA77 + seed) & 0xFFFFFFFF)
+
+
+def test_sr_hash_matches_independent_restatement():
+    rng = np.random.default_rng(0)
+    for seed, cell, step in zip(rng.integers(0, 2**32, 50), rng.integers(0, 2**32, 50),
+                                rng.integers(0, 2**31, 50)):
+        assert oracle.sr_hash(int(seed), int(cell), int(step)) == int(
+            _sr_hash_np(int(seed), int(cell), int(step)))
+    # avalanche: neighbouring cells / steps give unrelated low 16 bits
+    lo = np.array([oracle.sr_hash(7, c, 3) & 0xFFFF for c in range(4096)])
+    assert 0.45 < (lo < 32768).mean() < 0.55 and len(set(lo.tolist())) > 3900
+
+
+def _sr_bf16_np(f, step, seed):
+    u = f.view(np.uint32).ravel().astype(np.uint64)
+    h = _sr_hash_np(seed, np.arange(u.size, dtype=np.uint64), step) & 0xFFFF
+    return (((u + h) >> 16) & 0xFFFF).astype(np.uint16).reshape(f.shape)
+
+
+def test_bf16_sr_run_matches_numpy_restatement():
+    shape = (9, 7, 5)                      # (nz, ny, nx) of a 5x7x9 grid
+    rng = np.random.default_rng(3)
+    f0 = rng.random(shape, dtype=np.float32)
+    bits = oracle.to_lp(f0, 0)
+    c = np.array([0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4], np.float32)
+    want = bits.copy()
+    for t in range(6):
+        a = oracle.from_lp(want, 0)
+        want = _sr_bf16_np(oracle.run_f32(a, c, 1), 10 + t, 99)
+    got = oracle.run_bf16_sr(bits, c, 6, 99, step0=10)
+    assert np.array_equal(got, want)
+    # split runs continue the step numbering; an off-by-one step changes bits
+    half = oracle.run_bf16_sr(oracle.run_bf16_sr(bits, c, 2, 99, step0=10), c, 4, 99, step0=12)
+    assert np.array_equal(half, want)
+    assert not np.array_equal(oracle.run_bf16_sr(bits, c, 6, 99, step0=11), want)
+
+
+def test_bf16_sr_rounds_to_a_neighbour_without_bias():
+    rng = np.random.default_rng(4)
+    x = (rng.random(200000) * 2 - 1).astype(np.float32) * np.float32(0.3)
+    r = oracle.from_lp(_sr_bf16_np(x, 0, 5), 0)
+    lo = (x.view(np.uint32) & 0xFFFF0000).view(np.float32)        # toward zero
+    hi = ((x.view(np.uint32) & 0xFFFF0000) + 0x10000).view(np.float32)
+    assert np.all((r == lo) | (r == hi))
+    # exactly representable values are kept
+    xr = oracle.from_lp(oracle.to_lp(x, 0), 0)
+    assert np.array_equal(oracle.from_lp(_sr_bf16_np(xr, 1, 5), 0), xr)
+    # unbiased: one value rounded at many cells averages back to itself
+    v = np.full(400000, np.float32(0.1234567), np.float32)
+    m = oracle.from_lp(_sr_bf16_np(v, 2, 9), 0).astype(np.float64).mean()
+    ulp = 2.0 ** -11                                                 # bf16 ulp at 0.12
+    assert abs(m - 0.1234567) < 5 * ulp / np.sqrt(v.size)
+
+
+def test_bf16_sr_tracks_fp64_where_nearest_stalls():
+    shape = (48, 40, 56)
+    nx, ny, nz = shape
+    h = 1.0 / max(shape)
+    dt = 0.1 * h * h / 0.1
+    ce, cn, ct = (np.float32(0.1 * dt * n * n) for n in shape)
+    c = np.array([ce, ce, cn, cn, ct, ct, 1 - 2 * (ce + cn + ct)], np.float32)
+    k = oracle.constants_f32(*shape)
+    f0 = oracle.add_noise_f32(oracle.initialize_f32(*shape, k), 7, 1e-3)
+    ref = oracle.run_f64(f0.astype(np.float64), c.astype(np.float64), 200)
+    bits = oracle.to_lp(f0, 0)
+    rne = oracle.from_lp(oracle.run_lp(bits, c, 200, 0), 0)
+    sr = oracle.from_lp(oracle.run_bf16_sr(bits, c, 200, 1), 0)
+    def rel(a):
+        return np.linalg.norm(a - ref) / np.linalg.norm(ref)
+    print("bf16 rel L2 vs fp64: nearest %.3e stochastic %.3e" % (rel(rne), rel(sr)))
+    assert rel(rne) > 0.05 and rel(sr) < 5e-3
