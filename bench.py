@@ -149,7 +149,8 @@ def bench_batched(args, d3, ctx, dname):
     dtype = {"f32": d3.F32, "bf16": d3.BF16}[dname]
     depth = args.depth if args.depth is not None else 2
     arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
-    probs = workloads.batched_problems(dtype=dtype)
+    rounding = d3.ROUND_STOCHASTIC if args.bf16_rounding == "stochastic" else d3.ROUND_NEAREST
+    probs = workloads.batched_problems(dtype=dtype, rounding=rounding)
     inits = [workloads.initial_field(p, i) for i, p in enumerate(probs)]
     cells = sum(p["nx"] * p["ny"] * p["nz"] for p in probs)
     steps = probs[0]["steps"]
@@ -181,7 +182,8 @@ def bench_batched(args, d3, ctx, dname):
     for i, p in enumerate(probs):
         with d3.Field(ctx, p["nx"], p["ny"], p["nz"], dtype) as f:
             f.upload(inits[i], d3.F32)
-            f.run(p["steps"], p["coef"], arith=arith)
+            f.run(p["steps"], p["coef"], arith=arith, rounding=p.get("rounding", 0),
+                  seed=p.get("seed", 0))
             o = f.download(d3.F32)
         same = same and np.array_equal(o.view(np.uint32), outs[i].view(np.uint32))
     t_base = time.perf_counter() - t2
@@ -193,6 +195,7 @@ def bench_batched(args, d3, ctx, dname):
         "warmup": 1, "ms_per_step": round(t_e2e * 1e3 / steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": dname,
         "data": "synthetic (seeded dims log-uniform [32,256], cosine product + 1e-3 noise)",
+        "bf16_rounding": args.bf16_rounding if dtype == d3.BF16 else None,
         "config": {"workload": "%d grids %s, %d steps, depth %d, %s" % (n, dname.upper(), steps, depth,
                                                                           args.arith),
                    "cells": cells, "parallelism": "single GPU"},
@@ -211,7 +214,8 @@ def bench_served(args, d3, ctx):
     depth = args.depth if args.depth is not None else 2
     nreq = args.requests
     arena = workloads.PinnedArena()
-    base = workloads.served_requests(nreq, lam=1.0)
+    rounding = d3.ROUND_STOCHASTIC if args.bf16_rounding == "stochastic" else d3.ROUND_NEAREST
+    base = workloads.served_requests(nreq, lam=1.0, rounding=rounding)
     ins, outs = [], []
     for i, p in enumerate(base):
         a = arena.empty((p["nz"], p["ny"], p["nx"]))
@@ -234,7 +238,7 @@ def bench_served(args, d3, ctx):
     # open loop at each arrival rate
     rates = []
     for lam in [float(x) for x in args.rates.split(",")]:
-        reqs = workloads.served_requests(nreq, lam=lam)
+        reqs = workloads.served_requests(nreq, lam=lam, rounding=rounding)
         lat = []
         t_start = time.perf_counter()
         ids = []
@@ -259,7 +263,8 @@ def bench_served(args, d3, ctx):
     for i, p in enumerate(base):
         with d3.Field(ctx, p["nx"], p["ny"], p["nz"], p["dtype"]) as f:
             f.upload(ins[i], d3.F32)
-            f.run(p["steps"], p["coef"], arith=arith)
+            f.run(p["steps"], p["coef"], arith=arith, rounding=p.get("rounding", 0),
+                  seed=p.get("seed", 0))
             o = f.download(d3.F32)
         same = same and np.array_equal(o.view(np.uint32), server_outs[i].view(np.uint32))
     t_base = time.perf_counter() - t1
@@ -275,6 +280,7 @@ def bench_served(args, d3, ctx):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/bf16 storage, f32 compute",
         "data": "synthetic (seeded requests: edge log-uniform [32,384], slab aspect, steps U[50,2000], "
                 "70% FP32 / 30% BF16)",
+        "bf16_rounding": args.bf16_rounding,
         "config": {"workload": "%d requests, depth %d, %d steps per sweep, %s" % (
                        nreq, depth, args.sweep_steps, args.arith),
                    "cell_updates": work, "parallelism": "single GPU"},
@@ -303,6 +309,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=500, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU oracle comparison")
+    ap.add_argument("--bf16-rounding", default="stochastic", choices=["stochastic", "nearest"],
+                    help="rounding of BF16 storage in the batched and served workloads")
     ap.add_argument("--requests", type=int, default=150, help="served: requests per run")
     ap.add_argument("--rates", default="50,100,150", help="served: Poisson arrival rates (1/s)")
     ap.add_argument("--sweep-steps", type=int, default=32, help="served: steps per scheduling round")

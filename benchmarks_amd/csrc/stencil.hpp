@@ -49,6 +49,17 @@ template <> struct Traits<kF16> { using T = uint16_t; using C = float; };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values to packed BF16, round to nearest even: gfx950's
+// v_cvt_pk_bf16_f32.  It equals to_bf16 below on every one of the 2^32 input
+// patterns (NaNs quieted by setting bit 6, subnormals rounded, not flushed),
+// checked exhaustively on an MI355X.
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float lo, float hi) {
+  const f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
 
 // Buffer descriptor word 3 for untyped (raw) buffer access on the gfx9
 // family: DATA_FORMAT = 32 bit, so range checking is per dword.
@@ -142,7 +153,7 @@ struct Pack {
       }
     } else if constexpr (S == kBF16) {
 #pragma unroll
-      for (int i = 0; i < kWords; ++i) w[i] = to_bf16(v[2 * i]) | (to_bf16(v[2 * i + 1]) << 16);
+      for (int i = 0; i < kWords; ++i) w[i] = cvt_pk_bf16(v[2 * i], v[2 * i + 1]);
     } else {
 #pragma unroll
       for (int i = 0; i < kWords; ++i) w[i] = to_f16(v[2 * i]) | (to_f16(v[2 * i + 1]) << 16);
@@ -176,15 +187,16 @@ struct Pack {
   }
 };
 
-// Deterministic stochastic rounding FP32 -> BF16.  A 32-bit hash of (seed,
-// linear cell index, step) supplies 16 random bits that are added below the
-// BF16 mantissa before truncation, so x rounds away from zero with
+// Deterministic stochastic rounding FP32 -> BF16.  16 random bits are added
+// below the BF16 mantissa before truncation, so x rounds away from zero with
 // probability equal to its distance from the lower-magnitude neighbour (in
 // units of that ulp): unbiased, and reproducible on any device or on the
-// host.  The hash is the "lowbias32" integer mixer (C. Wellons, hash-prospector)
-// applied to a Weyl-style combination of the three inputs.
-__device__ __forceinline__ uint32_t sr_hash(uint32_t seed, uint32_t cell, uint32_t step) {
-  uint32_t x = cell * 0x9E3779B1u + step * 0x85EBCA77u + seed;
+// host.  The bits come from a 32-bit hash of (seed, cell pair, step): cells
+// x = 2i and 2i+1 of a row share pair i (pair index (z*ny + y) * ceil(nx/2) + i)
+// and take the low and high halves.  The hash is the "lowbias32" integer mixer
+// (C. Wellons, hash-prospector) of x = pair * 0x9E3779B1 + step * 0x85EBCA77
+// + seed; sr_mix gets that sum.
+__device__ __forceinline__ uint32_t sr_mix(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
   x ^= x >> 15;
@@ -192,16 +204,17 @@ __device__ __forceinline__ uint32_t sr_hash(uint32_t seed, uint32_t cell, uint32
   x ^= x >> 16;
   return x;
 }
-__device__ __forceinline__ float sr_bf16(float x, uint32_t h) {
+// r: 16 random bits.
+__device__ __forceinline__ float sr_bf16(float x, uint32_t r) {
   const uint32_t u = __float_as_uint(x);
   if ((u & 0x7f800000u) == 0x7f800000u) return __uint_as_float(to_bf16(x) << 16);  // inf / nan
-  return __uint_as_float((u + (h & 0xffffu)) & 0xffff0000u);
+  return __uint_as_float((u + r) & 0xffff0000u);
 }
 
 // Round a compute value to storage precision and back (identity for f32/f64).
 template <int S>
 __device__ __forceinline__ typename Traits<S>::C quantize(typename Traits<S>::C x) {
-  if constexpr (S == kBF16) return __uint_as_float(to_bf16(x) << 16);
+  if constexpr (S == kBF16) return __uint_as_float(cvt_pk_bf16(x, 0.0f) << 16);
   else if constexpr (S == kF16) return f16_bits_to_f32(to_f16(x));
   else return x;
 }
@@ -463,14 +476,20 @@ struct TileRunner {
         const uint32_t step = (uint32_t)(geo.step0 + s - 1);
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-          const uint32_t row = ((uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r)) * (uint32_t)geo.nx;
+          const uint32_t prow =
+              ((uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r)) * ((uint32_t)(geo.nx + 1) >> 1);
+          const uint32_t sbase = step * 0x85EBCA77u + geo.seed;
 #pragma unroll
-          for (int g = 0; g < G; ++g)
+          for (int g = 0; g < G; ++g) {
+            // xlane is a multiple of VEC (even), so a lane's pairs are its own
+            const uint32_t x0 = (prow + (uint32_t)(xlane + g * GSTRIDE) / 2u) * 0x9E3779B1u + sbase;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) {
-              const uint32_t cell = row + (uint32_t)(xlane + g * GSTRIDE + v);
-              out[r][g * VEC + v] = sr_bf16(out[r][g * VEC + v], sr_hash(geo.seed, cell, step));
+            for (int v = 0; v < VEC; v += 2) {
+              const uint32_t h = sr_mix(x0 + (uint32_t)(v / 2) * 0x9E3779B1u);
+              out[r][g * VEC + v] = sr_bf16(out[r][g * VEC + v], h & 0xffffu);
+              out[r][g * VEC + v + 1] = sr_bf16(out[r][g * VEC + v + 1], h >> 16);
             }
+          }
         }
       }
     }

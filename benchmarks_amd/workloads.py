@@ -20,23 +20,30 @@ from . import _native as N
 from .engine import add_noise_f32, constants_f32, initialize_f32, stable_coefficients
 
 
-def batched_problems(count=512, lo=32, hi=256, steps=200, dtype=N.F32, seed=4):
+def batched_problems(count=512, lo=32, hi=256, steps=200, dtype=N.F32, seed=4,
+                     rounding=N.ROUND_NEAREST):
+    """rounding applies to BF16 problems (ROUND_STOCHASTIC: seeded per problem)."""
     rng = np.random.default_rng(seed)
     out = []
-    for _ in range(count):
+    for i in range(count):
         nx, ny, nz = [int(np.exp(rng.uniform(np.log(lo), np.log(hi)))) for _ in range(3)]
         kappa = float(rng.uniform(0.05, 0.2))
         coef, dt = stable_coefficients(nx, ny, nz, kappa)
-        out.append({"nx": nx, "ny": ny, "nz": nz, "dtype": dtype, "steps": steps,
-                    "coef": coef, "kappa": kappa, "dt": dt})
+        p = {"nx": nx, "ny": ny, "nz": nz, "dtype": dtype, "steps": steps,
+             "coef": coef, "kappa": kappa, "dt": dt}
+        if dtype == N.BF16 and rounding == N.ROUND_STOCHASTIC:
+            p.update(rounding=rounding, seed=0x5EED0000 + i)
+        out.append(p)
     return out
 
 
-def served_requests(count, lam, lo=32, hi=384, s_lo=50, s_hi=2000, seed=5):
+def served_requests(count, lam, lo=32, hi=384, s_lo=50, s_hi=2000, seed=5,
+                    rounding=N.ROUND_NEAREST):
+    """rounding applies to the BF16 requests (ROUND_STOCHASTIC: seeded per request)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
-    for _ in range(count):
+    for i in range(count):
         t += float(rng.exponential(1.0 / lam))
         e = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         ay, az = np.exp(rng.uniform(np.log(0.5), np.log(2.0), 2))
@@ -46,8 +53,11 @@ def served_requests(count, lam, lo=32, hi=384, s_lo=50, s_hi=2000, seed=5):
         steps = int(rng.integers(s_lo, s_hi + 1))
         dtype = N.BF16 if rng.random() < 0.3 else N.F32
         coef, dt = stable_coefficients(nx, ny, nz)
-        out.append({"nx": nx, "ny": ny, "nz": nz, "dtype": dtype, "steps": steps, "coef": coef,
-                    "dt": dt, "arrival_s": t})
+        p = {"nx": nx, "ny": ny, "nz": nz, "dtype": dtype, "steps": steps, "coef": coef,
+             "dt": dt, "arrival_s": t}
+        if dtype == N.BF16 and rounding == N.ROUND_STOCHASTIC:
+            p.update(rounding=rounding, seed=0x5EED0000 + i)
+        out.append(p)
     return out
 
 

@@ -260,9 +260,9 @@ void oracle_run_lp(uint16_t *f1, uint16_t *f2, int nx, int ny, int nz,
 
 /* Stochastic rounding to BF16 (no reference counterpart).  The hash is
  * "lowbias32" (C. Wellons, hash-prospector) of a Weyl-style combination of the
- * seed, the linear cell index and the step number. */
-uint32_t oracle_sr_hash(uint32_t seed, uint32_t cell, uint32_t step) {
-  uint32_t x = cell * 0x9E3779B1u + step * 0x85EBCA77u + seed;
+ * seed, the index of the cell's x pair and the step number. */
+uint32_t oracle_sr_hash(uint32_t seed, uint32_t pair, uint32_t step) {
+  uint32_t x = pair * 0x9E3779B1u + step * 0x85EBCA77u + seed;
   x ^= x >> 16;
   x *= 0x7feb352du;
   x ^= x >> 15;
@@ -277,17 +277,20 @@ void oracle_run_bf16_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORAC
   float *a = (float *)malloc(sizeof(float) * (size_t)n);
   float *b = (float *)malloc(sizeof(float) * (size_t)n);
   oracle_lp_to_f32(f1, a, n, 0);
+  const uint32_t half_nx = (uint32_t)(nx + 1) / 2u;
   for (int i = 0; i < count; ++i) {
     oracle_step_f32(a, b, nx, ny, nz, c);
     const uint32_t step = (uint32_t)(step0 + i);
     for (long j = 0; j < n; ++j) {
+      const uint32_t x = (uint32_t)(j % nx), row = (uint32_t)(j / nx);
       uint32_t u;
       memcpy(&u, &b[j], 4);
       uint16_t h;
       if ((u & 0x7f800000u) == 0x7f800000u) {
         h = to_bf16(b[j]);
       } else {
-        h = (uint16_t)((u + (oracle_sr_hash(seed, (uint32_t)j, step) & 0xffffu)) >> 16);
+        const uint32_t r = oracle_sr_hash(seed, row * half_nx + x / 2u, step);
+        h = (uint16_t)((u + ((x & 1u) ? r >> 16 : r & 0xffffu)) >> 16);
       }
       a[j] = from_bf16(h);
     }

@@ -57,12 +57,14 @@ void oracle_run_lp(uint16_t *f1, uint16_t *f2, int nx, int ny, int nz,
                    const float c[ORACLE_NCOEF], int count, int kind);
 void oracle_f32_to_lp(const float *in, uint16_t *out, long n, int kind);
 /* BF16 storage with deterministic stochastic rounding (no reference
- * counterpart): after step t (numbered from step0), cell j's FP32 value u gets
- * the low 16 bits of lowbias32(j * 0x9E3779B1 + t * 0x85EBCA77 + seed) added to
- * its bit pattern before truncation to BF16; inf/nan round to nearest. */
+ * counterpart): after step t (numbered from step0), the FP32 value u of cell
+ * (x, y, z) gets 16 random bits added to its bit pattern before truncation to
+ * BF16 (inf/nan round to nearest).  The bits are the low (x even) or high
+ * (x odd) half of h = lowbias32(pair * 0x9E3779B1 + t * 0x85EBCA77 + seed),
+ * pair = (z*ny + y) * ceil(nx/2) + x/2; oracle_sr_hash returns h. */
 void oracle_run_bf16_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF],
                         int count, uint32_t seed, int step0);
-uint32_t oracle_sr_hash(uint32_t seed, uint32_t cell, uint32_t step);
+uint32_t oracle_sr_hash(uint32_t seed, uint32_t pair, uint32_t step);
 void oracle_lp_to_f32(const uint16_t *in, float *out, long n, int kind);
 
 /* Legacy accuracy: RMS vs the analytical field with a float accumulator in

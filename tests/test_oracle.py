@@ -279,9 +279,9 @@ def _lowbias32_np(x):
     return x
 
 
-def _sr_hash_np(seed, cell, step):
-    cell = np.asarray(cell, dtype=np.uint64)
-    return _lowbias32_np((cell * 0x9E3779B1 + step * 0x85EBCA77 + seed) & 0xFFFFFFFF)
+def _sr_hash_np(seed, pair, step):
+    pair = np.asarray(pair, dtype=np.uint64)
+    return _lowbias32_np((pair * 0x9E3779B1 + step * 0x85EBCA77 + seed) & 0xFFFFFFFF)
 
 
 def test_sr_hash_matches_independent_restatement():
@@ -290,15 +290,23 @@ def test_sr_hash_matches_independent_restatement():
                                 rng.integers(0, 2**31, 50)):
         assert oracle.sr_hash(int(seed), int(cell), int(step)) == int(
             _sr_hash_np(int(seed), int(cell), int(step)))
-    # avalanche: neighbouring cells / steps give unrelated low 16 bits
-    lo = np.array([oracle.sr_hash(7, c, 3) & 0xFFFF for c in range(4096)])
-    assert 0.45 < (lo < 32768).mean() < 0.55 and len(set(lo.tolist())) > 3900
+    # neighbouring pairs give unrelated 16-bit halves
+    h = np.array([oracle.sr_hash(7, c, 3) for c in range(4096)], np.uint64)
+    for half in (h & 0xFFFF, h >> 16):
+        assert 0.45 < (half < 32768).mean() < 0.55 and len(set(half.tolist())) > 3900
+    assert abs(np.corrcoef((h & 0xFFFF).astype(float), (h >> 16).astype(float))[0, 1]) < 0.05
 
 
 def _sr_bf16_np(f, step, seed):
-    u = f.view(np.uint32).ravel().astype(np.uint64)
-    h = _sr_hash_np(seed, np.arange(u.size, dtype=np.uint64), step) & 0xFFFF
-    return (((u + h) >> 16) & 0xFFFF).astype(np.uint16).reshape(f.shape)
+    """Stochastic rounding of an (nz, ny, nx) float32 array (or a 1-D row)."""
+    f = np.atleast_3d(f) if f.ndim != 3 else f
+    nx = f.shape[-1]
+    u = f.view(np.uint32).reshape(-1, nx).astype(np.uint64)
+    x = np.arange(nx, dtype=np.uint64)
+    pair = np.arange(u.shape[0], dtype=np.uint64)[:, None] * ((nx + 1) // 2) + x[None, :] // 2
+    h = _sr_hash_np(seed, pair, step)
+    r = np.where(x[None, :] & 1, h >> 16, h & 0xFFFF)
+    return (((u + r) >> 16) & 0xFFFF).astype(np.uint16).reshape(f.shape)
 
 
 def test_bf16_sr_run_matches_numpy_restatement():
@@ -321,7 +329,7 @@ def test_bf16_sr_run_matches_numpy_restatement():
 
 def test_bf16_sr_rounds_to_a_neighbour_without_bias():
     rng = np.random.default_rng(4)
-    x = (rng.random(200000) * 2 - 1).astype(np.float32) * np.float32(0.3)
+    x = ((rng.random(200000) * 2 - 1).astype(np.float32) * np.float32(0.3)).reshape(1, 1, -1)
     r = oracle.from_lp(_sr_bf16_np(x, 0, 5), 0)
     lo = (x.view(np.uint32) & 0xFFFF0000).view(np.float32)        # toward zero
     hi = ((x.view(np.uint32) & 0xFFFF0000) + 0x10000).view(np.float32)
@@ -330,7 +338,7 @@ def test_bf16_sr_rounds_to_a_neighbour_without_bias():
     xr = oracle.from_lp(oracle.to_lp(x, 0), 0)
     assert np.array_equal(oracle.from_lp(_sr_bf16_np(xr, 1, 5), 0), xr)
     # unbiased: one value rounded at many cells averages back to itself
-    v = np.full(400000, np.float32(0.1234567), np.float32)
+    v = np.full((1, 1, 400000), np.float32(0.1234567), np.float32)
     m = oracle.from_lp(_sr_bf16_np(v, 2, 9), 0).astype(np.float64).mean()
     ulp = 2.0 ** -11                                                 # bf16 ulp at 0.12
     assert abs(m - 0.1234567) < 5 * ulp / np.sqrt(v.size)
