@@ -204,11 +204,13 @@ __device__ __forceinline__ uint32_t sr_mix(uint32_t x) {
   x ^= x >> 16;
   return x;
 }
-// r: 16 random bits.
+// r: 16 random bits.  Infinities pass unchanged through the add-and-truncate
+// (zero mantissa, r < 2^16); NaNs are quieted like to_bf16 does.
 __device__ __forceinline__ float sr_bf16(float x, uint32_t r) {
   const uint32_t u = __float_as_uint(x);
-  if ((u & 0x7f800000u) == 0x7f800000u) return __uint_as_float(to_bf16(x) << 16);  // inf / nan
-  return __uint_as_float((u + r) & 0xffff0000u);
+  const uint32_t t = (u + r) & 0xffff0000u;
+  const uint32_t q = (u & 0xffff0000u) | 0x00400000u;  // to_bf16 of a NaN, in place
+  return __uint_as_float((u & 0x7fffffffu) > 0x7f800000u ? q : t);
 }
 
 // Round a compute value to storage precision and back (identity for f32/f64).
