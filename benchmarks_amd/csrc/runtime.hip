@@ -420,10 +420,18 @@ analytical_kernel(const void *f, int dt, int p, int nx, int ny, long long rows, 
 }
 
 static int ensure_partials(d3h_context *ctx) {
-  if (ctx->partials) return D3H_OK;
-  ctx->partial_blocks = std::max(1, ctx->num_cus) * 4;
-  D3H_HIP(hipMalloc(&ctx->partials, sizeof(double) * 4 * ctx->partial_blocks));
-  D3H_HIP(hipHostMalloc(&ctx->host_partials, sizeof(double) * 4 * ctx->partial_blocks));
+  if (ctx->partials && ctx->host_partials) return D3H_OK;
+  const int nb = std::max(1, ctx->num_cus) * 4;
+  void *dev = nullptr, *host = nullptr;
+  if (hipMalloc(&dev, sizeof(double) * 4 * nb) != hipSuccess)
+    return fail(D3H_ERR_NOMEM, "reduction scratch");
+  if (hipHostMalloc(&host, sizeof(double) * 4 * nb, hipHostMallocDefault) != hipSuccess) {
+    (void)hipFree(dev);
+    return fail(D3H_ERR_NOMEM, "pinned reduction scratch");
+  }
+  ctx->partials = static_cast<double *>(dev);
+  ctx->host_partials = static_cast<double *>(host);
+  ctx->partial_blocks = nb;
   return D3H_OK;
 }
 
@@ -571,17 +579,22 @@ int d3h_field_create(d3h_context *ctx, int nx, int ny, int nz, int dtype, d3h_fi
   f->plane = (size_t)f->pitch * ny;
   f->elems = f->plane * nz;
   f->bytes = f->elems * dtype_bytes(dtype);
+  const size_t bytes = f->bytes;
   for (int i = 0; i < 2; ++i) {
-    if (hipMalloc(&f->buf[i], f->bytes) != hipSuccess) {
+    if (hipMalloc(&f->buf[i], bytes) != hipSuccess) {
       d3h_field_destroy(f);
-      return fail(D3H_ERR_NOMEM, "hipMalloc(%zu) for a %dx%dx%d field failed", f->bytes, nx, ny, nz);
+      return fail(D3H_ERR_NOMEM, "hipMalloc(%zu) for a %dx%dx%d field failed", bytes, nx, ny, nz);
     }
-    if (hipMemsetAsync(f->buf[i], 0, f->bytes, ctx->stream) != hipSuccess) {
+    if (hipMemsetAsync(f->buf[i], 0, bytes, ctx->stream) != hipSuccess) {
       d3h_field_destroy(f);
       return fail(D3H_ERR_HIP, "hipMemsetAsync failed");
     }
   }
-  D3H_HIP(hipStreamSynchronize(ctx->stream));
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    d3h_field_destroy(f);
+    return fail(D3H_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+  }
   *out = f;
   return D3H_OK;
 }
