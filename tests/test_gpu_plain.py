@@ -237,3 +237,46 @@ def test_driver_autotune_knobs(tmp_path):
                        text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stderr
     assert re.search(r"Accuracy\s+:\s+(\S+)", r.stdout).group(1) == "7.227031e-07", r.stdout
+
+
+def test_full_size_512_bitwise_and_fp64_bound(ctx):
+    """The bench's 512^3 grid (FP32 and FP64 fields): 10 EXACT steps bitwise
+    equal to the oracle (OpenMP on the host), and the FP32 result within 1e-6
+    relative L2 of the FP64 GPU run of the same input."""
+    shape = (512, 512, 512)
+    k = d3.constants_f32(*shape)
+    f0 = d3.initialize_f32(*shape, k)
+    print("512^3 oracle ...", flush=True)
+    want = oracle.run_f32(f0, k[8:15], 10)
+    with d3.Field(ctx, *shape, d3.F32) as f, d3.Field(ctx, *shape, d3.F64) as g:
+        f.upload(f0)
+        f.run(10, d3.coefficients(k))
+        out = f.download()
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+        del out, want
+        g.upload(f0)
+        g.run(10, d3.coefficients(k))
+        c = f.compare(g)
+    print("512^3 fp32 vs fp64 after 10 steps:", c, flush=True)
+    assert c["rel_l2"] < 1e-6
+
+
+def test_full_size_768_depths_equal_plain(ctx):
+    """The temporally blocked 768^3 configuration (noisy input): depths 2, 4
+    and 8 equal plain steps bitwise at full size (device-side comparison;
+    the plain kernel itself is pinned to the oracle bitwise above)."""
+    shape = (768, 768, 768)
+    k = d3.constants_f32(*shape)
+    f0 = d3.add_noise_f32(d3.initialize_f32(*shape, k), 768, 1e-3)
+    steps = 9
+    with d3.Field(ctx, *shape) as plain:
+        plain.upload(f0)
+        plain.run(steps, d3.coefficients(k), depth=1)
+        for depth in (2, 4, 8):
+            with d3.Field(ctx, *shape) as tb:
+                tb.upload(f0)
+                tm = tb.run(steps, d3.coefficients(k), depth=depth)
+                assert tm["steps"] == steps
+                c = tb.compare(plain)
+                print("768^3 depth", depth, c, flush=True)
+                assert c["max_abs"] == 0.0, depth
