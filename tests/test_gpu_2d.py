@@ -228,3 +228,23 @@ def test_row_tiles_refuse_3d_operator(ctx):
         assert d3.tiles()[tm["tile"]]["row_stream"] == 0
         out = f.download()
     assert np.array_equal(out.view(np.uint32), oracle.run_f32(f0, coef, 6).view(np.uint32))
+
+
+def test_full_size_16384_depths_equal_plain(ctx):
+    """The 2D bench grid (16384^2): every row-streaming depth equals plain
+    steps bitwise at full size (device-side comparison)."""
+    n = 16384
+    k = d3.constants2d_f32(n, n)
+    f0 = d3.initialize2d_f32(n, n, k)
+    c = d3.coefficients(k)
+    with d3.Field(ctx, n, n, 1) as plain:
+        plain.set_y_reversed(True)
+        plain.upload(f0)
+        plain.run(11, c, depth=1)
+        for depth in (2, 4, 8):
+            with d3.Field(ctx, n, n, 1) as tb:
+                tb.set_y_reversed(True)
+                tb.upload(f0)
+                tm = tb.run(11, c, depth=depth)
+                assert d3.tiles()[tm["tile"]]["row_stream"] == 1
+                assert tb.compare(plain)["max_abs"] == 0.0, depth
