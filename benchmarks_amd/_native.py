@@ -69,6 +69,8 @@ SIGNATURES = {
     "run_kernel_hip": (c_int, [c_int, P(c_float), c_int, c_int, c_int] + [c_float] * 7),
     "finalize_benchmark_hip": (c_int, []),
     "kernel_time_hip": (c_int, [P(c_float)]),
+    "diffusion_hip_": (c_int, [P(c_float), P(c_float), P(c_int), P(c_int), P(c_int)] + [P(c_float)] * 7
+                       + [P(c_int)]),
     "d3h_status_string": (c_char_p, [c_int]),
     "d3h_last_error": (c_char_p, []),
     "d3h_context_create": (c_int, [c_int, P(c_void_p)]),

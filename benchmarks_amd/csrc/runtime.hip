@@ -1106,4 +1106,39 @@ int kernel_time_hip(float *seconds) {
   return D3H_OK;
 }
 
+// By-reference form of the reference's Fortran variant.  One context and
+// field are kept for the process and re-created when the dimensions change;
+// calls must not overlap.
+static struct ByRef {
+  d3h_context *ctx = nullptr;
+  d3h_field *field = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+} g_byref;
+
+int diffusion_hip_(float *f1, float *f2, int *nx, int *ny, int *nz, float *ce, float *cw, float *cn,
+                   float *cs, float *ct, float *cb, float *cc, int *count) {
+  (void)f2;
+  if (!f1 || !nx || !ny || !nz || !ce || !cw || !cn || !cs || !ct || !cb || !cc || !count)
+    return fail(D3H_ERR_INVALID, "NULL argument");
+  if (*count < 0) return fail(D3H_ERR_INVALID, "count %d < 0", *count);
+  if (!g_byref.ctx) D3H_TRY(d3h_context_create(env_int("D3H_DEVICE", 0), &g_byref.ctx));
+  if (g_byref.field && (g_byref.nx != *nx || g_byref.ny != *ny || g_byref.nz != *nz)) {
+    d3h_field_destroy(g_byref.field);
+    g_byref.field = nullptr;
+  }
+  if (!g_byref.field) {
+    D3H_TRY(d3h_field_create(g_byref.ctx, *nx, *ny, *nz, D3H_F32, &g_byref.field));
+    g_byref.nx = *nx; g_byref.ny = *ny; g_byref.nz = *nz;
+  }
+  const double coef[7] = {*ce, *cw, *cn, *cs, *ct, *cb, *cc};
+  d3h_run_config cfg;
+  d3h_run_config_default(&cfg);
+  cfg.temporal_depth = env_int("D3H_TEMPORAL_DEPTH", 1);
+  cfg.arith = env_int("D3H_ARITH", D3H_ARITH_EXACT);
+  cfg.use_graph = env_int("D3H_USE_GRAPH", 0);
+  D3H_TRY(d3h_field_upload(g_byref.field, f1, D3H_F32));
+  D3H_TRY(d3h_field_run(g_byref.field, *count, coef, &cfg, nullptr));
+  return d3h_field_download(g_byref.field, f1, D3H_F32);
+}
+
 }  // extern "C"

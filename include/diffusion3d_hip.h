@@ -73,6 +73,15 @@ int run_kernel_hip(int count, float *f1_host, int nx, int ny, int nz,
 int finalize_benchmark_hip(void);
 /* Kernel-only seconds of the last run_kernel_hip (events around the launches). */
 int kernel_time_hip(float *seconds);
+/* By-reference entry in the shape of the reference's Fortran variant
+ * (diffusion3d/diffusion3d_fortran.h:9-12; called as
+ * diffusion_fortran_(f1_, f2_, &nx_, ..., &ce_, ..., &cc_, &count) from
+ * Diffusion3DFortran::RunKernel).  f1 is advanced by exactly *count steps
+ * (the Fortran variant runs 2 * (count / 2)); f2 is not touched.  The context
+ * and device field are kept between calls (re-created when the dimensions
+ * change); calls must not overlap.  Same environment knobs as run_kernel_hip. */
+int diffusion_hip_(float *f1, float *f2, int *nx, int *ny, int *nz, float *ce, float *cw,
+                   float *cn, float *cs, float *ct, float *cb, float *cc, int *count);
 
 /* ---- errors ---------------------------------------------------------------- */
 const char *d3h_status_string(int status);

@@ -5,6 +5,7 @@ reference's own build) and to the oracle; FMA mode must stay within 1e-6
 relative L2 (FP32).  FP64 is checked against the oracle's FP64 instantiation
 (bitwise in EXACT mode; the stated tolerance is 1e-13 relative L2).
 """
+import ctypes
 import hashlib
 import os
 import re
@@ -280,3 +281,25 @@ def test_full_size_768_depths_equal_plain(ctx):
                 c = tb.compare(plain)
                 print("768^3 depth", depth, c, flush=True)
                 assert c["max_abs"] == 0.0, depth
+
+
+def test_fortran_shaped_by_reference_entry():
+    """diffusion_hip_ in the Fortran variant's calling convention: f1 advanced
+    by exactly `count` steps (odd counts too), bitwise as the reference's CPU
+    path; dimension changes re-create the device field."""
+    L = N.lib()
+    for shape, count in (((40, 30, 20), 7), ((33, 17, 5), 100), ((40, 30, 20), 2)):
+        k = d3.constants_f32(*shape)
+        f1 = np.ascontiguousarray(d3.initialize_f32(*shape, k))
+        f2 = np.zeros_like(f1)
+        nx, ny, nz = (ctypes.c_int(v) for v in shape)
+        cs = [ctypes.c_float(float(v)) for v in k[8:15]]
+        cnt = ctypes.c_int(count)
+        fp = N.P(ctypes.c_float)
+        rc = L.diffusion_hip_(f1.ctypes.data_as(fp), f2.ctypes.data_as(fp), ctypes.byref(nx),
+                              ctypes.byref(ny), ctypes.byref(nz), *[ctypes.byref(c) for c in cs],
+                              ctypes.byref(cnt))
+        assert rc == 0, N.lib().d3h_last_error()
+        want = oracle.run_f32(d3.initialize_f32(*shape, k), k[8:15], count)
+        assert np.array_equal(f1.view(np.uint32), want.view(np.uint32)), (shape, count)
+        assert not f2.any()
