@@ -361,3 +361,28 @@ def test_bf16_sr_tracks_fp64_where_nearest_stalls():
         return np.linalg.norm(a - ref) / np.linalg.norm(ref)
     print("bf16 rel L2 vs fp64: nearest %.3e stochastic %.3e" % (rel(rne), rel(sr)))
     assert rel(rne) > 0.05 and rel(sr) < 5e-3
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (12, 10, 7), (64, 32, 16)])
+def test_fp64_constants_and_field_closed_form(shape):
+    """The FP64 instantiation (no reference build): constants from their
+    formulas, and the cosine product against numpy's float64 evaluation."""
+    nx, ny, nz = shape
+    k = oracle.constants_f64(*shape)
+    dx, dy, dz = 1.0 / nx, 1.0 / ny, 1.0 / nz
+    dt = 0.1 * dx * dx / 0.1
+    want = [0.1, dx, dy, dz, 2 * np.pi, 2 * np.pi, 2 * np.pi, dt, 0.1 * dt / dx ** 2,
+            0.1 * dt / dx ** 2, 0.1 * dt / dy ** 2, 0.1 * dt / dy ** 2, 0.1 * dt / dz ** 2,
+            0.1 * dt / dz ** 2]
+    assert np.allclose(k[:14], want, rtol=1e-15, atol=0)
+    assert k[14] == 1.0 - (k[8] + k[9] + k[10] + k[11] + k[12] + k[13])
+    t = 37 * dt
+    f = oracle.initialize_f64(*shape, k, t)
+    a = np.exp(-0.1 * t * (2 * np.pi) ** 2)
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dy
+    z = (np.arange(nz) + 0.5) * dz
+    ref = (0.125 * (1 - a * np.cos(2 * np.pi * x))[None, None, :]
+           * (1 - a * np.cos(2 * np.pi * y))[None, :, None]
+           * (1 - a * np.cos(2 * np.pi * z))[:, None, None])
+    assert np.max(np.abs(f - ref)) < 4e-17
