@@ -127,6 +127,10 @@ def cpu_baseline(nx, ny, nz, steps_sample):
                       % (nx, ny, nz, steps_sample, seconds)}
 
 
+def depth_desc(depth):
+    return ("depth %d" % depth) if depth else "depth per size class (tuned, else FP32 2 / BF16 1)"
+
+
 def fp64_errors(d3, ctx, problems, inits, outs, sample):
     """Relative L2 / max-abs of results against the FP64 GPU run of the same
     problem (exact arithmetic), over a sample of problems."""
@@ -147,7 +151,9 @@ def fp64_errors(d3, ctx, problems, inits, outs, sample):
 def bench_batched(args, d3, ctx, dname):
     from benchmarks_amd import workloads
     dtype = {"f32": d3.F32, "bf16": d3.BF16}[dname]
-    depth = args.depth if args.depth is not None else 2
+    # 0: per size class, the autotuner's choice or the dtype's default (2 for
+    # FP32, 1 for BF16 storage)
+    depth = args.depth if args.depth is not None else 0
     arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
     rounding = d3.ROUND_STOCHASTIC if args.bf16_rounding == "stochastic" else d3.ROUND_NEAREST
     probs = workloads.batched_problems(dtype=dtype, rounding=rounding)
@@ -196,7 +202,7 @@ def bench_batched(args, d3, ctx, dname):
         "scaling": "weak", "vs_baseline": None, "dtype": dname,
         "data": "synthetic (seeded dims log-uniform [32,256], cosine product + 1e-3 noise)",
         "bf16_rounding": args.bf16_rounding if dtype == d3.BF16 else None,
-        "config": {"workload": "%d grids %s, %d steps, depth %d, %s" % (n, dname.upper(), steps, depth,
+        "config": {"workload": "%d grids %s, %d steps, %s, %s" % (n, dname.upper(), steps, depth_desc(depth),
                                                                           args.arith),
                    "cells": cells, "parallelism": "single GPU"},
         "device_resident_solves_per_s": round(n / t_dev, 2),
@@ -211,7 +217,7 @@ def bench_batched(args, d3, ctx, dname):
 def bench_served(args, d3, ctx):
     from benchmarks_amd import workloads
     arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
-    depth = args.depth if args.depth is not None else 2
+    depth = args.depth if args.depth is not None else 0
     nreq = args.requests
     arena = workloads.PinnedArena()
     rounding = d3.ROUND_STOCHASTIC if args.bf16_rounding == "stochastic" else d3.ROUND_NEAREST
@@ -281,8 +287,8 @@ def bench_served(args, d3, ctx):
         "data": "synthetic (seeded requests: edge log-uniform [32,384], slab aspect, steps U[50,2000], "
                 "70% FP32 / 30% BF16)",
         "bf16_rounding": args.bf16_rounding,
-        "config": {"workload": "%d requests, depth %d, %d steps per sweep, %s" % (
-                       nreq, depth, args.sweep_steps, args.arith),
+        "config": {"workload": "%d requests, %s, %d steps per sweep, %s" % (
+                       nreq, depth_desc(depth), args.sweep_steps, args.arith),
                    "cell_updates": work, "parallelism": "single GPU"},
         "closed_loop_gcell_updates_per_s": round(work / t_closed / 1e9, 2),
         "per_request_plain_solves_per_s": round(nreq / t_base, 2),

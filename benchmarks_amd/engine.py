@@ -303,7 +303,7 @@ class Batch:
               "d3h_batch_download")
         return out
 
-    def run(self, arith=N.ARITH_EXACT, depth=2, graph=True):
+    def run(self, arith=N.ARITH_EXACT, depth=0, graph=True):
         t = N.Timing()
         check(lib().d3h_batch_run(self.handle, arith, depth, 1 if graph else 0, ctypes.byref(t)),
               "d3h_batch_run")
@@ -339,7 +339,7 @@ def stable_coefficients(nx, ny, nz, kappa=0.1, dtype=np.float32):
 class Server:
     """Continuous-batching request server (d3h_server_*)."""
 
-    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0, arith=N.ARITH_EXACT, depth=2,
+    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0, arith=N.ARITH_EXACT, depth=0,
                  sweep_steps=8):
         cfg = N.ServerConfig(hbm_budget_bytes, max_active, arith, depth, sweep_steps)
         h = ctypes.c_void_p()

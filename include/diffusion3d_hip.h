@@ -240,7 +240,8 @@ int d3h_batch_upload(d3h_batch *b, int i, const void *host, int host_dtype);
 int d3h_batch_download(d3h_batch *b, int i, void *host, int host_dtype);
 /* Run every problem to its step count.  arith as above; temporal_depth 1, 2,
  * 4 or 8, or 0 = the depth d3h_autotune(..., 0, ...) chose for each problem's
- * size class, else 2; use_graph 1 = capture + replay. */
+ * size class, else 2 (FP32/FP64 storage) or 1 (BF16/FP16 storage);
+ * use_graph 1 = capture + replay. */
 int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph,
                   d3h_timing *timing);
 
@@ -255,7 +256,8 @@ typedef struct {
   int max_active;           /* max concurrently resident requests; 0 = 4096     */
   int arith;                /* D3H_ARITH_EXACT / D3H_ARITH_FMA                   */
   int temporal_depth;       /* steps per sweep launch: 1, 2 or 4; 0 = the tuned
-                               depth of each request's size class, else 2     */
+                               depth of each request's size class, else 2
+                               (FP32/FP64) or 1 (BF16/FP16)                    */
   int sweep_steps;          /* steps per scheduling round; 0 = 8                 */
 } d3h_server_config;
 
