@@ -29,8 +29,8 @@ int check_problem(const d3h_problem &p) {
   if (std::fabs(sum - 1.0) > 1e-5) return fail(D3H_ERR_UNSTABLE, "coefficients sum to %.9g, not 1", sum);
   if (p.rounding != D3H_ROUND_NEAREST && p.rounding != D3H_ROUND_STOCHASTIC)
     return fail(D3H_ERR_INVALID, "rounding %d", p.rounding);
-  if (p.rounding == D3H_ROUND_STOCHASTIC && p.dtype != D3H_BF16)
-    return fail(D3H_ERR_UNSUPPORTED, "stochastic rounding is for BF16 storage (dtype %d)", p.dtype);
+  if (p.rounding == D3H_ROUND_STOCHASTIC && p.dtype != D3H_BF16 && p.dtype != D3H_F16)
+    return fail(D3H_ERR_UNSUPPORTED, "stochastic rounding is for 16-bit storage (dtype %d)", p.dtype);
   return D3H_OK;
 }
 

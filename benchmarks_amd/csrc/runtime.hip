@@ -681,8 +681,9 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
   if (cfg.rounding != D3H_ROUND_NEAREST && cfg.rounding != D3H_ROUND_STOCHASTIC)
     return fail(D3H_ERR_INVALID, "rounding %d", cfg.rounding);
   const int sr = cfg.rounding == D3H_ROUND_STOCHASTIC;
-  if (sr && f->dtype != D3H_BF16)
-    return fail(D3H_ERR_UNSUPPORTED, "stochastic rounding is for BF16 storage (field dtype %d)", f->dtype);
+  if (sr && f->dtype != D3H_BF16 && f->dtype != D3H_F16)
+    return fail(D3H_ERR_UNSUPPORTED, "stochastic rounding is for 16-bit storage (field dtype %d)",
+                f->dtype);
   if (sr && f->steps_done + count > INT_MAX)
     return fail(D3H_ERR_INVALID, "step numbers of stochastic rounding exceed 2^31");
   D3H_TRY(resolve_entry(ctx, f->nx, f->ny, f->nz, f->dtype, cfg, rows_ok(f, coef), &e, &zchunks));

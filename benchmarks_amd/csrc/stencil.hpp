@@ -89,7 +89,7 @@ struct Geometry {
   long long plane;
   int ntx, nty, nzc;
   int ox, oy, oz, hx;
-  // Stochastic rounding of BF16 storage (sr = 1): every stored time level is
+  // Stochastic rounding of BF16/FP16 storage (sr = 1): every stored time level is
   // rounded with a hash of (seed, cell, step); step0 is the number of steps
   // the field had taken before this launch (a batched launch adds launch * T).
   int sr;
@@ -211,6 +211,18 @@ __device__ __forceinline__ float sr_bf16(float x, uint32_t r) {
   const uint32_t t = (u + r) & 0xffff0000u;
   const uint32_t q = (u & 0xffff0000u) | 0x00400000u;  // to_bf16 of a NaN, in place
   return __uint_as_float((u & 0x7fffffffu) > 0x7f800000u ? q : t);
+}
+
+// Stochastic rounding FP32 -> FP16 with the same 16 random bits r: inside the
+// FP16 normal range (2^-14 <= |x| < 2^16) the low 13 bits of r are added below
+// the FP16 mantissa before truncation (the result is exactly representable,
+// or 2^16 which converts to inf); outside it (zero, subnormal, inf, nan) the
+// value rounds to nearest even.
+__device__ __forceinline__ float sr_f16(float x, uint32_t r) {
+  const uint32_t u = __float_as_uint(x);
+  const uint32_t e = (u >> 23) & 0xffu;
+  const float t = __uint_as_float((u + (r & 0x1fffu)) & 0xffffe000u);
+  return f16_bits_to_f32(to_f16((e >= 113u && e <= 142u) ? t : x));
 }
 
 // Round a compute value to storage precision and back (identity for f32/f64).
@@ -473,7 +485,7 @@ struct TileRunner {
         }
       }
     }
-    if constexpr (S == kBF16) {
+    if constexpr (S == kBF16 || S == kF16) {
       if (geo.sr) {
         const uint32_t step = (uint32_t)(geo.step0 + s - 1);
 #pragma unroll
@@ -488,8 +500,13 @@ struct TileRunner {
 #pragma unroll
             for (int v = 0; v < VEC; v += 2) {
               const uint32_t h = sr_mix(x0 + (uint32_t)(v / 2) * 0x9E3779B1u);
-              out[r][g * VEC + v] = sr_bf16(out[r][g * VEC + v], h & 0xffffu);
-              out[r][g * VEC + v + 1] = sr_bf16(out[r][g * VEC + v + 1], h >> 16);
+              if constexpr (S == kBF16) {
+                out[r][g * VEC + v] = sr_bf16(out[r][g * VEC + v], h & 0xffffu);
+                out[r][g * VEC + v + 1] = sr_bf16(out[r][g * VEC + v + 1], h >> 16);
+              } else {
+                out[r][g * VEC + v] = sr_f16(out[r][g * VEC + v], h & 0xffffu);
+                out[r][g * VEC + v + 1] = sr_f16(out[r][g * VEC + v + 1], h >> 16);
+              }
             }
           }
         }

@@ -44,11 +44,12 @@ enum { D3H_F32 = 0, D3H_F64 = 1, D3H_BF16 = 2, D3H_F16 = 3 };
  * way a contracting compiler does it, fma(cc,c,cw*w) then a chain of fmas. */
 enum { D3H_ARITH_EXACT = 0, D3H_ARITH_FMA = 1 };
 
-/* Rounding of BF16 storage.  NEAREST: round to nearest even after every step.
- * STOCHASTIC: round up or down with probability given by the distance to the
- * two neighbours, using a deterministic hash of (seed, linear cell index,
- * step number) -- unbiased, so the small per-step changes of this problem are
- * not lost to rounding, and reproducible bitwise.  Step numbers count from the
+/* Rounding of 16-bit storage.  NEAREST: round to nearest even after every
+ * step.  STOCHASTIC: round up or down with probability given by the distance
+ * to the two neighbours, using a deterministic hash of (seed, cell, step
+ * number) -- unbiased, so the small per-step changes of this problem are not
+ * lost to rounding, and reproducible bitwise.  For FP16 it applies in the FP16
+ * normal range; zeros, subnormals, inf and nan round to nearest.  Step numbers count from the
  * field's last upload, from 0 in every d3h_batch_run, and from a server
  * request's first step. */
 enum { D3H_ROUND_NEAREST = 0, D3H_ROUND_STOCHASTIC = 1 };
@@ -115,7 +116,7 @@ typedef struct {
   int tile;            /* kernel/tile index (d3h_tile_info); -1 = tuned choice */
   int zchunks;         /* z-chunks per launch; 0 = the tuned / default value  */
   int use_graph;       /* 1: capture the launch sequence in a hipGraph         */
-  int rounding;        /* D3H_ROUND_*; STOCHASTIC needs a BF16 field           */
+  int rounding;        /* D3H_ROUND_*; STOCHASTIC needs a BF16 or FP16 field   */
   unsigned int seed;   /* seed of stochastic rounding                          */
 } d3h_run_config;
 
@@ -228,7 +229,7 @@ typedef struct {
   int dtype;        /* D3H_F32 / D3H_BF16 / D3H_F16 / D3H_F64 */
   int steps;        /* exact number of steps for this problem */
   double coef[7];   /* ce cw cn cs ct cb cc                    */
-  int rounding;     /* D3H_ROUND_*; STOCHASTIC needs BF16      */
+  int rounding;     /* D3H_ROUND_*; STOCHASTIC: BF16 or FP16   */
   unsigned int seed;
 } d3h_problem;
 

@@ -46,6 +46,8 @@ def lib():
             "oracle_run_bf16_sr": [_u16, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int,
                                    ctypes.c_uint32, ctypes.c_int],
             "oracle_sr_hash": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32],
+            "oracle_run_lp_sr": [_u16, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_uint32, ctypes.c_int],
             "oracle_lp_to_f32": [_u16, _f, ctypes.c_long, ctypes.c_int],
             "oracle_accuracy_f32": [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int],
             "oracle_rms_vs_analytical_f32": [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f,
@@ -158,6 +160,16 @@ def run_bf16_sr(bits, coef, count, seed, step0=0):
     c = np.ascontiguousarray(coef, np.float32)
     nz, ny, nx = a.shape
     lib().oracle_run_bf16_sr(_p(a, _u16), nx, ny, nz, _p(c, _f), count, seed, step0)
+    return a
+
+
+def run_lp_sr(bits, coef, count, kind, seed, step0=0):
+    """`count` steps of 16-bit storage (kind 0 BF16, 1 FP16) with stochastic
+    rounding (new array)."""
+    a = np.array(bits, dtype=np.uint16, copy=True, order="C")
+    c = np.ascontiguousarray(coef, np.float32)
+    nz, ny, nx = a.shape
+    lib().oracle_run_lp_sr(_p(a, _u16), nx, ny, nz, _p(c, _f), count, kind, seed, step0)
     return a
 
 

@@ -271,33 +271,41 @@ uint32_t oracle_sr_hash(uint32_t seed, uint32_t pair, uint32_t step) {
   return x;
 }
 
-void oracle_run_bf16_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF],
-                        int count, uint32_t seed, int step0) {
+void oracle_run_lp_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF], int count,
+                      int kind, uint32_t seed, int step0) {
   const long n = (long)nx * ny * nz;
   float *a = (float *)malloc(sizeof(float) * (size_t)n);
   float *b = (float *)malloc(sizeof(float) * (size_t)n);
-  oracle_lp_to_f32(f1, a, n, 0);
+  oracle_lp_to_f32(f1, a, n, kind);
   const uint32_t half_nx = (uint32_t)(nx + 1) / 2u;
   for (int i = 0; i < count; ++i) {
     oracle_step_f32(a, b, nx, ny, nz, c);
     const uint32_t step = (uint32_t)(step0 + i);
     for (long j = 0; j < n; ++j) {
       const uint32_t x = (uint32_t)(j % nx), row = (uint32_t)(j / nx);
-      uint32_t u;
-      memcpy(&u, &b[j], 4);
-      uint16_t h;
-      if ((u & 0x7f800000u) == 0x7f800000u) {
-        h = to_bf16(b[j]);
+      const uint32_t h = oracle_sr_hash(seed, row * half_nx + x / 2u, step);
+      const uint32_t r = (x & 1u) ? h >> 16 : h & 0xffffu;
+      const uint32_t u = f2u(b[j]);
+      if (kind == 0) {
+        uint16_t v;
+        if ((u & 0x7f800000u) == 0x7f800000u) v = to_bf16(b[j]);
+        else v = (uint16_t)((u + r) >> 16);
+        a[j] = from_bf16(v);
       } else {
-        const uint32_t r = oracle_sr_hash(seed, row * half_nx + x / 2u, step);
-        h = (uint16_t)((u + ((x & 1u) ? r >> 16 : r & 0xffffu)) >> 16);
+        const uint32_t e = (u >> 23) & 0xffu;
+        const float t = u2f((u + (r & 0x1fffu)) & 0xffffe000u);
+        a[j] = from_f16(to_f16((e >= 113u && e <= 142u) ? t : b[j]));
       }
-      a[j] = from_bf16(h);
     }
   }
-  oracle_f32_to_lp(a, f1, n, 0);
+  oracle_f32_to_lp(a, f1, n, kind);
   free(a);
   free(b);
+}
+
+void oracle_run_bf16_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF],
+                        int count, uint32_t seed, int step0) {
+  oracle_run_lp_sr(f1, nx, ny, nz, c, count, 0, seed, step0);
 }
 
 /* ---------------------------------------------------------------------------

@@ -64,6 +64,11 @@ void oracle_f32_to_lp(const float *in, uint16_t *out, long n, int kind);
  * pair = (z*ny + y) * ceil(nx/2) + x/2; oracle_sr_hash returns h. */
 void oracle_run_bf16_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF],
                         int count, uint32_t seed, int step0);
+/* kind 0: as oracle_run_bf16_sr.  kind 1 (FP16 storage): with the same 16
+ * bits r, values of FP32 exponent 113..142 (2^-14 <= |u| < 2^16) get r & 0x1fff
+ * added below the FP16 mantissa before truncation; others round to nearest. */
+void oracle_run_lp_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF], int count,
+                      int kind, uint32_t seed, int step0);
 uint32_t oracle_sr_hash(uint32_t seed, uint32_t pair, uint32_t step);
 void oracle_lp_to_f32(const uint16_t *in, float *out, long n, int kind);
 

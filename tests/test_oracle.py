@@ -386,3 +386,53 @@ def test_fp64_constants_and_field_closed_form(shape):
            * (1 - a * np.cos(2 * np.pi * y))[None, :, None]
            * (1 - a * np.cos(2 * np.pi * z))[:, None, None])
     assert np.max(np.abs(f - ref)) < 4e-17
+
+
+def _sr_f16_np(f, step, seed):
+    """FP16 stochastic rounding of an (nz, ny, nx) float32 array -> float32
+    values (numpy restatement)."""
+    nx = f.shape[-1]
+    u = f.view(np.uint32).reshape(-1, nx).astype(np.uint64)
+    x = np.arange(nx, dtype=np.uint64)
+    pair = np.arange(u.shape[0], dtype=np.uint64)[:, None] * ((nx + 1) // 2) + x[None, :] // 2
+    h = _sr_hash_np(seed, pair, step)
+    r = np.where(x[None, :] & 1, h >> 16, h & 0xFFFF) & 0x1FFF
+    t = (((u + r) & 0xFFFFE000) & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+    e = (u >> 23) & 0xFF
+    src = np.where((e >= 113) & (e <= 142), t, f.reshape(-1, nx))
+    return src.astype(np.float16).astype(np.float32).reshape(f.shape)
+
+
+def test_f16_sr_run_matches_numpy_restatement():
+    rng = np.random.default_rng(8)
+    f0 = (rng.random((6, 9, 7), dtype=np.float32) * np.float32(0.5)).astype(np.float32)
+    f0[0, 0, :3] = [0.0, 3e-6, -2e-7]            # zero and FP16 subnormals: nearest even
+    bits = oracle.to_lp(f0, 1)
+    c = np.array([0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4], np.float32)
+    a = oracle.from_lp(bits, 1)
+    with np.errstate(over="ignore"):
+        for t in range(5):
+            a = _sr_f16_np(oracle.run_f32(a, c, 1), 3 + t, 17)
+    got = oracle.from_lp(oracle.run_lp_sr(bits, c, 5, 1, 17, step0=3), 1)
+    assert np.array_equal(got.view(np.uint32), a.view(np.uint32))
+    # kind 0 through the general entry equals the BF16 function
+    b16 = oracle.to_lp(f0, 0)
+    assert np.array_equal(oracle.run_lp_sr(b16, c, 4, 0, 5), oracle.run_bf16_sr(b16, c, 4, 5))
+
+
+def test_f16_sr_tracks_fp64_where_nearest_stalls():
+    shape = (48, 40, 56)
+    h = 1.0 / max(shape)
+    dt = 0.1 * h * h / 0.1
+    ce, cn, ct = (np.float32(0.1 * dt * n * n) for n in shape)
+    c = np.array([ce, ce, cn, cn, ct, ct, 1 - 2 * (ce + cn + ct)], np.float32)
+    k = oracle.constants_f32(*shape)
+    f0 = oracle.add_noise_f32(oracle.initialize_f32(*shape, k), 7, 1e-3)
+    ref = oracle.run_f64(f0.astype(np.float64), c.astype(np.float64), 400)
+    bits = oracle.to_lp(f0, 1)
+    rne = oracle.from_lp(oracle.run_lp(bits, c, 400, 1), 1)
+    sr = oracle.from_lp(oracle.run_lp_sr(bits, c, 400, 1, 3), 1)
+    def rel(a):
+        return np.linalg.norm(a - ref) / np.linalg.norm(ref)
+    print("f16 rel L2 vs fp64: nearest %.3e stochastic %.3e" % (rel(rne), rel(sr)))
+    assert rel(sr) < rel(rne) / 5 and rel(sr) < 1e-3
