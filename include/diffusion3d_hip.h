@@ -147,7 +147,9 @@ int d3h_field_download(d3h_field *f, void *host, int host_dtype);
  * diffusion3d does (diffusion3d/baseline.cc:39-40); the diffusion/ family
  * names y the other way round and sums y-1 first (diffusion/baseline.h:59-67),
  * so its results are reproduced bitwise on a reversed field.  The stencil is
- * symmetric in y, so only the rounding order changes. */
+ * symmetric in y, so only the rounding order changes.  The flag applies to
+ * later uploads and downloads (set it before the first upload); device-side
+ * comparisons and norms see the device order. */
 int d3h_field_set_y_reversed(d3h_field *f, int reversed);
 /* Advance the field by exactly `count` steps on the device (synchronous). */
 int d3h_field_run(d3h_field *f, int count, const double coef[7],
