@@ -107,6 +107,8 @@ SIGNATURES = {
     "d3h_batch_destroy": (c_int, [c_void_p]),
     "d3h_batch_upload": (c_int, [c_void_p, c_int, c_void_p, c_int]),
     "d3h_batch_download": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "d3h_batch_upload_all": (c_int, [c_void_p, P(c_void_p), c_int]),
+    "d3h_batch_download_all": (c_int, [c_void_p, P(c_void_p), c_int]),
     "d3h_batch_run": (c_int, [c_void_p, c_int, c_int, c_int, P(Timing)]),
     "d3h_server_create": (c_int, [c_void_p, P(ServerConfig), P(c_void_p)]),
     "d3h_server_destroy": (c_int, [c_void_p]),

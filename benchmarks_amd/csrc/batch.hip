@@ -334,6 +334,55 @@ int d3h_batch_download(d3h_batch *b, int i, void *host, int host_dtype) {
   return D3H_OK;
 }
 
+// Whole-batch transfers: every problem gets its own dense staging slot, all
+// copies and conversions are queued on the stream, and the call synchronises
+// once (pinned host buffers make the copies asynchronous DMA).
+static int staging_slots(d3h_batch *b, int host_dtype, std::vector<size_t> *offs) {
+  size_t total = 0;
+  offs->clear();
+  for (const auto &p : b->probs) {
+    offs->push_back(total);
+    total += ((size_t)p.nx * p.ny * p.nz * dtype_bytes(host_dtype) + 255) / 256 * 256;
+  }
+  return batch_staging(b, std::max<size_t>(total, 256));
+}
+
+int d3h_batch_upload_all(d3h_batch *b, const void *const *hosts, int host_dtype) {
+  if (!b || !hosts || !dtype_valid(host_dtype)) return fail(D3H_ERR_INVALID, "bad upload arguments");
+  for (size_t i = 0; i < b->probs.size(); ++i)
+    if (!hosts[i]) return fail(D3H_ERR_INVALID, "host buffer %zu is NULL", i);
+  D3H_HIP(hipSetDevice(b->ctx->device));
+  std::vector<size_t> offs;
+  D3H_TRY(staging_slots(b, host_dtype, &offs));
+  hipStream_t st = b->ctx->stream;
+  for (size_t i = 0; i < b->probs.size(); ++i) {
+    const d3h_problem &p = b->probs[i];
+    char *dst = static_cast<char *>(b->pool) + b->off[i] + b->cur[i] * b->bytes[i];
+    D3H_TRY(copy_in(hosts[i], host_dtype, dst, p.dtype, b->pitch[i], p.nx, p.ny, p.nz,
+                    static_cast<char *>(b->staging) + offs[i], st));
+  }
+  D3H_HIP(hipStreamSynchronize(st));
+  return D3H_OK;
+}
+
+int d3h_batch_download_all(d3h_batch *b, void *const *hosts, int host_dtype) {
+  if (!b || !hosts || !dtype_valid(host_dtype)) return fail(D3H_ERR_INVALID, "bad download arguments");
+  for (size_t i = 0; i < b->probs.size(); ++i)
+    if (!hosts[i]) return fail(D3H_ERR_INVALID, "host buffer %zu is NULL", i);
+  D3H_HIP(hipSetDevice(b->ctx->device));
+  std::vector<size_t> offs;
+  D3H_TRY(staging_slots(b, host_dtype, &offs));
+  hipStream_t st = b->ctx->stream;
+  for (size_t i = 0; i < b->probs.size(); ++i) {
+    const d3h_problem &p = b->probs[i];
+    const char *src = static_cast<const char *>(b->pool) + b->off[i] + b->cur[i] * b->bytes[i];
+    D3H_TRY(copy_out(src, p.dtype, b->pitch[i], hosts[i], host_dtype, p.nx, p.ny, p.nz,
+                     static_cast<char *>(b->staging) + offs[i], st));
+  }
+  D3H_HIP(hipStreamSynchronize(st));
+  return D3H_OK;
+}
+
 int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph, d3h_timing *timing) {
   const double t0 = now_ms();
   if (!b) return fail(D3H_ERR_INVALID, "batch is NULL");

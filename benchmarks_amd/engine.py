@@ -194,10 +194,14 @@ class Field:
         check(lib().d3h_field_set_y_reversed(self.handle, 1 if reversed_ else 0),
               "d3h_field_set_y_reversed")
 
-    def download(self, host_dtype=None):
+    def download(self, host_dtype=None, out=None):
         if host_dtype is None:
             host_dtype = self.dtype
-        out = np.empty((self.nz, self.ny, self.nx), dtype=NP_DTYPES[host_dtype])
+        if out is None:
+            out = np.empty((self.nz, self.ny, self.nx), dtype=NP_DTYPES[host_dtype])
+        elif (not out.flags.c_contiguous or out.dtype != np.dtype(NP_DTYPES[host_dtype])
+              or out.size != self.nx * self.ny * self.nz):
+            raise ValueError("out must be a dense array of the field's size and host dtype")
         check(lib().d3h_field_download(self.handle, _dptr(out), host_dtype), "d3h_field_download")
         return out
 
@@ -302,6 +306,26 @@ class Batch:
         check(lib().d3h_batch_download(self.handle, i, _dptr(out), host_dtype),
               "d3h_batch_download")
         return out
+
+    def _ptrs(self, arrays, host_dtype):
+        if len(arrays) != len(self.problems):
+            raise ValueError("%d arrays for %d problems" % (len(arrays), len(self.problems)))
+        for a, p in zip(arrays, self.problems):
+            if not a.flags.c_contiguous or a.dtype != np.dtype(NP_DTYPES[host_dtype]) or \
+                    a.size != p["nx"] * p["ny"] * p["nz"]:
+                raise ValueError("host arrays must be dense, of the problem's size and host dtype")
+        return (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+    def upload_all(self, hosts, host_dtype):
+        """Upload every problem's initial field in one call (one synchronisation)."""
+        check(lib().d3h_batch_upload_all(self.handle, self._ptrs(hosts, host_dtype), host_dtype),
+              "d3h_batch_upload_all")
+
+    def download_all(self, outs, host_dtype):
+        """Download every problem into the given dense arrays (one synchronisation)."""
+        check(lib().d3h_batch_download_all(self.handle, self._ptrs(outs, host_dtype), host_dtype),
+              "d3h_batch_download_all")
+        return outs
 
     def run(self, arith=N.ARITH_EXACT, depth=0, graph=True):
         t = N.Timing()

@@ -241,6 +241,12 @@ int d3h_batch_destroy(d3h_batch *b);
 /* Upload problem i's initial field (dense host array of host_dtype). */
 int d3h_batch_upload(d3h_batch *b, int i, const void *host, int host_dtype);
 int d3h_batch_download(d3h_batch *b, int i, void *host, int host_dtype);
+/* Every problem at once: hosts[i] is problem i's dense host array (all of
+ * host_dtype).  Copies and conversions are queued together and the call
+ * synchronises once; with pinned buffers (d3h_host_alloc) the copies overlap.
+ * Keeps a staging area of the batch's total dense size. */
+int d3h_batch_upload_all(d3h_batch *b, const void *const *hosts, int host_dtype);
+int d3h_batch_download_all(d3h_batch *b, void *const *hosts, int host_dtype);
 /* Run every problem to its step count.  arith as above; temporal_depth 1, 2,
  * 4 or 8, or 0 = the depth d3h_autotune(..., 0, ...) chose for each problem's
  * size class, else 2 (FP32/FP64 storage) or 1 (BF16/FP16 storage);

@@ -75,3 +75,44 @@ def test_batch_rejects_unstable_coefficients(ctx):
     with pytest.raises(d3.D3HError) as e:
         d3.Batch(ctx, [p])
     assert e.value.status == 7
+
+
+def test_whole_batch_transfers_equal_per_problem(ctx):
+    """upload_all / download_all (one synchronisation, per-problem staging
+    slots) move the same bytes as the per-problem calls, for pinned and
+    pageable buffers and a converting host dtype."""
+    from benchmarks_amd import workloads
+    rng = np.random.default_rng(31)
+    probs, inits = [], []
+    for i in range(9):
+        nx, ny, nz = [int(v) for v in rng.integers(5, 70, 3)]
+        coef, _ = d3.stable_coefficients(nx, ny, nz)
+        dt = [d3.F32, d3.BF16, d3.F64][i % 3]
+        probs.append({"nx": nx, "ny": ny, "nz": nz, "dtype": dt, "steps": int(rng.integers(1, 9)),
+                      "coef": coef})
+        k = d3.constants_f32(nx, ny, nz)
+        inits.append(d3.add_noise_f32(d3.initialize_f32(nx, ny, nz, k), 50 + i, 1e-3))
+    arena = workloads.PinnedArena()
+    try:
+        pinned = []
+        for f in inits:
+            a = arena.empty(f.shape)
+            a[...] = f
+            pinned.append(a)
+        with d3.Batch(ctx, probs) as a, d3.Batch(ctx, probs) as b:
+            for i, f in enumerate(inits):
+                a.upload(i, f, d3.F32)
+            b.upload_all(pinned, d3.F32)
+            a.run(depth=2)
+            b.run(depth=2)
+            outs = [arena.empty(f.shape) for f in inits]
+            b.download_all(outs, d3.F32)
+            pageable = b.download_all([np.empty_like(f) for f in inits], d3.F32)
+            for i in range(len(probs)):
+                want = a.download(i, d3.F32)
+                assert np.array_equal(outs[i].view(np.uint32), want.view(np.uint32)), i
+                assert np.array_equal(pageable[i].view(np.uint32), want.view(np.uint32)), i
+            with pytest.raises(ValueError):
+                b.download_all(outs[:-1], d3.F32)
+    finally:
+        arena.close()
