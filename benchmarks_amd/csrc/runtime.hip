@@ -666,6 +666,96 @@ int d3h_field_download(d3h_field *f, void *host, int host_dtype) {
   return D3H_OK;
 }
 
+// ---- raw field files ------------------------------------------------------------
+// 40-byte little-endian header ("D3HFIELD", version 1, dtype, nx, ny, nz,
+// flags, steps since upload), then the dense field in host row order.
+namespace {
+struct RawHeader {
+  char magic[8];
+  uint32_t version, dtype;
+  int32_t nx, ny, nz;
+  uint32_t flags;  // bit 0: the field was stored with reversed rows
+  uint64_t steps;
+};
+static_assert(sizeof(RawHeader) == 40, "raw header layout");
+constexpr char kRawMagic[8] = {'D', '3', 'H', 'F', 'I', 'E', 'L', 'D'};
+}  // namespace
+
+int d3h_raw_write(const char *path, const void *host, int dtype, int nx, int ny, int nz, uint64_t steps,
+                  int flags) {
+  if (!path || !host) return fail(D3H_ERR_INVALID, "path/host is NULL");
+  if (!dtype_valid(dtype) || nx <= 0 || ny <= 0 || nz <= 0)
+    return fail(D3H_ERR_INVALID, "dtype %d dims %dx%dx%d", dtype, nx, ny, nz);
+  RawHeader h{};
+  memcpy(h.magic, kRawMagic, 8);
+  h.version = 1;
+  h.dtype = (uint32_t)dtype;
+  h.nx = nx; h.ny = ny; h.nz = nz;
+  h.flags = (uint32_t)flags;
+  h.steps = steps;
+  const size_t bytes = (size_t)nx * ny * nz * dtype_bytes(dtype);
+  FILE *out = fopen(path, "wb");
+  if (!out) return fail(D3H_ERR_INVALID, "cannot write %s", path);
+  const bool ok = fwrite(&h, sizeof h, 1, out) == 1 && fwrite(host, 1, bytes, out) == bytes;
+  if (fclose(out) != 0 || !ok) return fail(D3H_ERR_INVALID, "write to %s failed", path);
+  return D3H_OK;
+}
+
+int d3h_raw_read(const char *path, void *host, size_t capacity, int *dtype, int *nx, int *ny, int *nz,
+                 uint64_t *steps, int *flags) {
+  if (!path) return fail(D3H_ERR_INVALID, "path is NULL");
+  FILE *in = fopen(path, "rb");
+  if (!in) return fail(D3H_ERR_NOT_FOUND, "cannot read %s", path);
+  RawHeader h{};
+  if (fread(&h, sizeof h, 1, in) != 1 || memcmp(h.magic, kRawMagic, 8) != 0 || h.version != 1 ||
+      !dtype_valid((int)h.dtype) || h.nx <= 0 || h.ny <= 0 || h.nz <= 0) {
+    fclose(in);
+    return fail(D3H_ERR_INVALID, "%s is not a field file", path);
+  }
+  if (dtype) *dtype = (int)h.dtype;
+  if (nx) *nx = h.nx;
+  if (ny) *ny = h.ny;
+  if (nz) *nz = h.nz;
+  if (steps) *steps = h.steps;
+  if (flags) *flags = (int)h.flags;
+  const size_t bytes = (size_t)h.nx * h.ny * h.nz * dtype_bytes((int)h.dtype);
+  if (!host) {  // header only
+    fclose(in);
+    return D3H_OK;
+  }
+  if (capacity < bytes) {
+    fclose(in);
+    return fail(D3H_ERR_INVALID, "%s needs %zu bytes, buffer has %zu", path, bytes, capacity);
+  }
+  const bool ok = fread(host, 1, bytes, in) == bytes;
+  fclose(in);
+  return ok ? D3H_OK : fail(D3H_ERR_INVALID, "%s is truncated", path);
+}
+
+int d3h_field_save(d3h_field *f, const char *path) {
+  if (!f || !path) return fail(D3H_ERR_INVALID, "field/path is NULL");
+  std::vector<char> host((size_t)f->nx * f->ny * f->nz * dtype_bytes(f->dtype));
+  D3H_TRY(d3h_field_download(f, host.data(), f->dtype));
+  return d3h_raw_write(path, host.data(), f->dtype, f->nx, f->ny, f->nz, (uint64_t)f->steps_done,
+                       f->yrev ? 1 : 0);
+}
+
+int d3h_field_load(d3h_field *f, const char *path) {
+  if (!f || !path) return fail(D3H_ERR_INVALID, "field/path is NULL");
+  int dt = 0, nx = 0, ny = 0, nz = 0;
+  uint64_t steps = 0;
+  D3H_TRY(d3h_raw_read(path, nullptr, 0, &dt, &nx, &ny, &nz, &steps, nullptr));
+  if (nx != f->nx || ny != f->ny || nz != f->nz)
+    return fail(D3H_ERR_INVALID, "%s holds %dx%dx%d, the field is %dx%dx%d", path, nx, ny, nz, f->nx,
+                f->ny, f->nz);
+  std::vector<char> host((size_t)nx * ny * nz * dtype_bytes(dt));
+  D3H_TRY(d3h_raw_read(path, host.data(), host.size(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr));
+  D3H_TRY(d3h_field_upload(f, host.data(), dt));
+  f->steps_done = (long long)steps;  // stochastic rounding continues the step numbers
+  return D3H_OK;
+}
+
 int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_config *cfg_in,
                   d3h_timing *timing) {
   const double t0 = now_ms();

@@ -89,6 +89,32 @@ def initialize2d_f32(nx, ny, constants, time=0.0, out=None):
     return out
 
 
+def save_raw(path, array, dtype=None, steps=0, y_reversed=False):
+    """Write a dense (nz, ny, nx) host array as a raw field file (bit-exact)."""
+    if dtype is None:
+        dtype = {np.dtype(np.float32): N.F32, np.dtype(np.float64): N.F64,
+                 np.dtype(np.float16): N.F16, np.dtype(np.uint16): N.BF16}[array.dtype]
+    a = np.ascontiguousarray(array)
+    if a.ndim != 3 or a.dtype != np.dtype(NP_DTYPES[dtype]):
+        raise ValueError("save_raw needs a (nz, ny, nx) array of the given dtype")
+    nz, ny, nx = a.shape
+    check(lib().d3h_raw_write(str(path).encode(), _dptr(a), dtype, nx, ny, nz, steps,
+                              1 if y_reversed else 0), "d3h_raw_write")
+
+
+def load_raw(path):
+    """Read a raw field file: returns (array (nz, ny, nx), info dict)."""
+    dt, nx, ny, nz, fl = (ctypes.c_int() for _ in range(5))
+    st = ctypes.c_uint64()
+    p = str(path).encode()
+    check(lib().d3h_raw_read(p, None, 0, ctypes.byref(dt), ctypes.byref(nx), ctypes.byref(ny),
+                             ctypes.byref(nz), ctypes.byref(st), ctypes.byref(fl)), "d3h_raw_read")
+    out = np.empty((nz.value, ny.value, nx.value), dtype=NP_DTYPES[dt.value])
+    check(lib().d3h_raw_read(p, _dptr(out), out.nbytes, None, None, None, None, None, None),
+          "d3h_raw_read")
+    return out, {"dtype": dt.value, "steps": st.value, "y_reversed": bool(fl.value & 1)}
+
+
 def add_noise_f32(buf, seed, amp):
     check(lib().d3h_add_noise_f32(buf.ctypes.data_as(N.P(ctypes.c_float)), buf.size, seed,
                                   float(amp)), "d3h_add_noise_f32")
@@ -187,6 +213,14 @@ class Field:
             raise ValueError("host array dtype %s does not match %s" % (host.dtype,
                                                                          DTYPE_NAMES[host_dtype]))
         check(lib().d3h_field_upload(self.handle, _dptr(host), host_dtype), "d3h_field_upload")
+
+    def save(self, path):
+        """Bit-exact raw field file (header + dense data in the field's dtype)."""
+        check(lib().d3h_field_save(self.handle, str(path).encode()), "d3h_field_save")
+
+    def load(self, path):
+        """Restore a field saved by save() (same dimensions)."""
+        check(lib().d3h_field_load(self.handle, str(path).encode()), "d3h_field_load")
 
     def set_y_reversed(self, reversed_=True):
         """Keep the rows of every plane reversed on the device so the y-1 term

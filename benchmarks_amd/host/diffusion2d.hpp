@@ -48,6 +48,7 @@ class Diffusion2DHIP {
     if (f1_) d3h_host_free(f1_);
   }
   std::string GetName() const { return "hip"; }
+  void SetRawDump(bool on) { raw_dump_ = on; }
   std::string GetDescription() const { return "HIP engine, 5-point operator on reversed rows"; }
 
   // diffusion/diffusion.h:112-141 with ndim = 2.
@@ -73,6 +74,11 @@ class Diffusion2DHIP {
     std::cout << "Benchmarking finished.\n";
     DisplayResult(count, elapsed_time);
     if (dump) Dump();
+    if (raw_dump_) {
+      const std::string path = std::string("diffusion_result.") + GetName() + ".raw";
+      CheckStatus(d3h_raw_write(path.c_str(), f1_, D3H_F32, nx_, ny_, 1, (uint64_t)count, 0),
+                  "d3h_raw_write");
+    }
   }
 
  private:
@@ -83,6 +89,7 @@ class Diffusion2DHIP {
   d3h_context *ctx_ = nullptr;
   d3h_field *field_ = nullptr;
   float kernel_seconds_ = 0.f;
+  bool raw_dump_ = false;
 
   size_t Cells() const { return (size_t)nx_ * ny_; }
 

@@ -76,6 +76,8 @@ class Diffusion3D {
   }
   virtual ~Diffusion3D() {}
   virtual std::string GetName() const = 0;
+  // Also write the final field bit-exact (d3h_raw_write format) after Dump.
+  void SetRawDump(bool on) { raw_dump_ = on; }
 
   // The reference's RunBenchmark (diffusion3d/diffusion3d.h:69-84) with the
   // warm-up of the diffusion/ family (diffusion/diffusion.h:126-131): run,
@@ -99,11 +101,14 @@ class Diffusion3D {
     std::cout << "Benchmarking finished.\n";
     DisplayResult(count, elapsed_time);
     if (dump) Dump();
+    if (raw_dump_) DumpRaw(count);
     FinalizeBenchmark();
   }
 
  protected:
+  bool raw_dump_ = false;
   std::string GetDumpPath() const { return std::string("diffusion3d_result.") + GetName() + ".out"; }
+  virtual void DumpRaw(int count) const = 0;
   virtual void InitializeBenchmark() = 0;
   virtual void ReinitializeInput() = 0;
   virtual void WarmingUp() { RunKernel(1); }
@@ -161,6 +166,12 @@ class Baseline : public Diffusion3D {
     return (REAL)sqrtf(err / len);
   }
   // "%.7E\n" per cell (diffusion3d/baseline.cc:62-70).
+  // diffusion3d_result.<name>.raw: the host field bit-exact.
+  void DumpRaw(int count) const override {
+    const std::string path = std::string("diffusion3d_result.") + GetName() + ".raw";
+    CheckStatus(d3h_raw_write(path.c_str(), f1_, D3H_F32, nx_, ny_, nz_, (uint64_t)count, 0),
+                "d3h_raw_write");
+  }
   void Dump() const override {
     FILE *out = fopen(GetDumpPath().c_str(), "w");
     if (!out) {

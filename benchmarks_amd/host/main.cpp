@@ -26,17 +26,18 @@ static void PrintUsage(std::ostream &os, const char *prog) {
      << "\t--ny N      Size of y (overrides --size)\n"
      << "\t--nz N      Size of z (overrides --size)\n"
      << "\t--dump      Dump the final data to file\n"
+     << "\t--dump-raw  Also write the final field bit-exact (d3h_raw_write format)\n"
      << "\t--warmup    Enable warming-up runs\n"
      << "\t--help      Display this help message\n";
 }
 
 int main(int argc, char *argv[]) {
   int size = 256, count = 100, nx = 0, ny = 0, nz = 0, nd = 3;
-  bool dump = false, warmup = false;
+  bool dump = false, warmup = false, dump_raw = false;
   static struct option long_options[] = {
       {"count", 1, 0, 0}, {"size", 1, 0, 0}, {"dump", 0, 0, 0}, {"help", 0, 0, 0},
       {"warmup", 0, 0, 0}, {"nx", 1, 0, 0}, {"ny", 1, 0, 0},   {"nz", 1, 0, 0},
-      {"nd", 1, 0, 0},     {0, 0, 0, 0}};
+      {"nd", 1, 0, 0},     {"dump-raw", 0, 0, 0}, {0, 0, 0, 0}};
   while (1) {
     int option_index = 0;
     int c = getopt_long(argc, argv, "", long_options, &option_index);
@@ -52,6 +53,7 @@ int main(int argc, char *argv[]) {
       case 6: ny = atoi(optarg); break;
       case 7: nz = atoi(optarg); break;
       case 8: nd = atoi(optarg); break;
+      case 9: dump_raw = true; break;
       default: break;
     }
   }
@@ -64,10 +66,12 @@ int main(int argc, char *argv[]) {
   }
   if (nd == 2) {
     diffusion::Diffusion2DHIP bmk(nx, ny);
+    bmk.SetRawDump(dump_raw);
     bmk.RunBenchmark(count, dump, warmup);
     return 0;
   }
   Diffusion3D *bmk = new Diffusion3DHIP(nx, ny, nz, argc, argv);
+  bmk->SetRawDump(dump_raw);
   bmk->RunBenchmark(count, dump, warmup);
   delete bmk;
   return 0;

@@ -142,6 +142,22 @@ int d3h_field_dims(const d3h_field *f, int *nx, int *ny, int *nz, int *dtype);
  * (converted on the device, round-to-nearest-even when narrowing). */
 int d3h_field_upload(d3h_field *f, const void *host, int host_dtype);
 int d3h_field_download(d3h_field *f, void *host, int host_dtype);
+/* Raw field files: a 40-byte header ("D3HFIELD", version, dtype, nx, ny, nz,
+ * flags, steps since upload) and the dense field in the field's dtype and the
+ * host's row order -- bit-exact, unlike the reference's "%.7E" text dump
+ * (diffusion3d/baseline.cc:62-70).  load needs a field of the same
+ * dimensions (the stored dtype is converted like an upload) and restores the
+ * step count, so stochastic rounding resumes where it stopped. */
+int d3h_field_save(d3h_field *f, const char *path);
+int d3h_field_load(d3h_field *f, const char *path);
+/* The same file format for host arrays (e.g. the driver's --dump-raw).
+ * d3h_raw_read with host == NULL reads the header only; otherwise it needs
+ * capacity >= nx*ny*nz*size(dtype).  Outputs may be NULL.  flags bit 0
+ * records reversed rows. */
+int d3h_raw_write(const char *path, const void *host, int dtype, int nx, int ny, int nz,
+                  uint64_t steps, int flags);
+int d3h_raw_read(const char *path, void *host, size_t capacity, int *dtype, int *nx, int *ny,
+                 int *nz, uint64_t *steps, int *flags);
 /* Store the rows of every plane in reverse order on the device (uploads and
  * downloads flip them).  The kernels sum the y+1 term before the y-1 term as
  * diffusion3d does (diffusion3d/baseline.cc:39-40); the diffusion/ family

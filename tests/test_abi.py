@@ -104,3 +104,30 @@ def test_driver_help():
     r = subprocess.run([N.DRIVER_PATH, "--help"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0
     assert "--count N" in r.stderr and "--size N" in r.stderr and "--dump" in r.stderr
+
+
+def test_raw_field_files_round_trip(tmp_path):
+    """d3h_raw_write / d3h_raw_read (host side, no GPU): bit-exact round trip
+    for every dtype, header fields, and errors for bad or short files."""
+    rng = np.random.default_rng(4)
+    arrays = [rng.random((3, 5, 7), dtype=np.float32), rng.random((2, 4, 9)),
+              rng.integers(0, 65535, (4, 3, 6)).astype(np.uint16),
+              rng.random((5, 2, 3)).astype(np.float16)]
+    for i, a in enumerate(arrays):
+        p = tmp_path / ("f%d.raw" % i)
+        d3.save_raw(p, a, steps=17 + i, y_reversed=bool(i % 2))
+        b, info = d3.load_raw(p)
+        assert b.dtype == a.dtype and b.shape == a.shape
+        assert np.array_equal(b.view(np.uint8), a.view(np.uint8))
+        assert info["steps"] == 17 + i and info["y_reversed"] == bool(i % 2)
+        assert p.stat().st_size == 40 + a.nbytes
+    bad = tmp_path / "bad.raw"
+    bad.write_bytes(b"NOTAFIELD" + bytes(64))
+    with pytest.raises(d3.D3HError):
+        d3.load_raw(bad)
+    short = tmp_path / "short.raw"
+    short.write_bytes((tmp_path / "f0.raw").read_bytes()[:100])
+    with pytest.raises(d3.D3HError):
+        d3.load_raw(short)
+    with pytest.raises(d3.D3HError):
+        d3.load_raw(tmp_path / "missing.raw")
