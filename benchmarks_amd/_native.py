@@ -108,6 +108,8 @@ SIGNATURES = {
     "d3h_constants2d_f32": (c_int, [c_int, c_int, P(c_float)]),
     "d3h_initialize2d_f32": (c_int, [P(c_float), c_int, c_int, P(c_float), c_float]),
     "d3h_add_noise_f32": (c_int, [P(c_float), c_size_t, c_uint64, c_float]),
+    "d3h_problem_check": (c_int, [P(Problem)]),
+    "d3h_problem_footprint": (c_int, [P(Problem), c_int, c_int, P(c_size_t)]),
     "d3h_batch_create": (c_int, [c_void_p, P(Problem), c_int, P(c_void_p)]),
     "d3h_batch_destroy": (c_int, [c_void_p]),
     "d3h_batch_upload": (c_int, [c_void_p, c_int, c_void_p, c_int]),

@@ -410,11 +410,30 @@ int d3h_server_destroy(d3h_server *s) {
   return D3H_OK;
 }
 
+int d3h_problem_check(const d3h_problem *p) {
+  if (!p) return fail(D3H_ERR_INVALID, "problem is NULL");
+  return check_problem(*p);
+}
+
+int d3h_problem_footprint(const d3h_problem *p, int host_dtype, int out_dtype, size_t *bytes) {
+  if (!p || !bytes) return fail(D3H_ERR_INVALID, "NULL argument");
+  if (!dtype_valid(host_dtype) || !dtype_valid(out_dtype)) return fail(D3H_ERR_INVALID, "dtype");
+  D3H_TRY(check_problem(*p));
+  const int pitch = pitch_elems(p->nx, p->dtype);
+  const size_t buf = align_up((size_t)pitch * p->ny * p->nz * dtype_bytes(p->dtype));
+  const size_t n = (size_t)p->nx * p->ny * p->nz;
+  size_t stage = 0;
+  if (needs_staging(host_dtype, p->dtype, pitch, p->nx)) stage = std::max(stage, n * dtype_bytes(host_dtype));
+  if (needs_staging(out_dtype, p->dtype, pitch, p->nx)) stage = std::max(stage, n * dtype_bytes(out_dtype));
+  *bytes = 2 * buf + align_up(stage);
+  return D3H_OK;
+}
+
 int d3h_server_submit(d3h_server *s, const d3h_problem *p, const void *host_in, int host_dtype,
                       void *host_out, int out_dtype, int64_t *id) {
   if (!s || !p || !host_in || !host_out || !id) return fail(D3H_ERR_INVALID, "NULL argument");
-  if (!dtype_valid(host_dtype) || !dtype_valid(out_dtype)) return fail(D3H_ERR_INVALID, "dtype");
-  D3H_TRY(check_problem(*p));
+  size_t footprint = 0;
+  D3H_TRY(d3h_problem_footprint(p, host_dtype, out_dtype, &footprint));
   auto *r = new Request();
   r->p = *p;
   r->in = host_in;
@@ -423,11 +442,7 @@ int d3h_server_submit(d3h_server *s, const d3h_problem *p, const void *host_in, 
   r->out_dt = out_dtype;
   r->pitch = pitch_elems(p->nx, p->dtype);
   r->buf_bytes = align_up((size_t)r->pitch * p->ny * p->nz * dtype_bytes(p->dtype));
-  const size_t n = (size_t)p->nx * p->ny * p->nz;
-  size_t stage = 0;
-  if (needs_staging(host_dtype, p->dtype, r->pitch, p->nx)) stage = std::max(stage, n * dtype_bytes(host_dtype));
-  if (needs_staging(out_dtype, p->dtype, r->pitch, p->nx)) stage = std::max(stage, n * dtype_bytes(out_dtype));
-  r->footprint = 2 * r->buf_bytes + align_up(stage);
+  r->footprint = footprint;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->fatal != D3H_OK) {
     delete r;

@@ -251,6 +251,13 @@ typedef struct {
   unsigned int seed;
 } d3h_problem;
 
+/* Host-only checks (no device needed).  d3h_problem_check: dims, dtype,
+ * steps >= 0, coefficients finite, >= 0 and summing to 1 (D3H_ERR_UNSTABLE
+ * otherwise), rounding valid for the dtype.  d3h_problem_footprint: device
+ * bytes a server request needs (two pitched buffers + conversion staging),
+ * which admission compares with the free part of the server's pool. */
+int d3h_problem_check(const d3h_problem *p);
+int d3h_problem_footprint(const d3h_problem *p, int host_dtype, int out_dtype, size_t *bytes);
 int d3h_batch_create(d3h_context *ctx, const d3h_problem *problems, int count,
                      d3h_batch **out);
 int d3h_batch_destroy(d3h_batch *b);

@@ -131,3 +131,47 @@ def test_raw_field_files_round_trip(tmp_path):
         d3.load_raw(short)
     with pytest.raises(d3.D3HError):
         d3.load_raw(tmp_path / "missing.raw")
+
+
+def _problem(nx, ny, nz, dtype=N.F32, steps=3, coef=(0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4),
+             rounding=0):
+    return N.Problem(nx, ny, nz, dtype, steps, (ctypes.c_double * 7)(*coef), rounding, 0)
+
+
+def test_problem_checks_on_the_host():
+    """Failure handling that needs no GPU: unstable or malformed problems come
+    back as status codes from the host-side check the batch and server use."""
+    L = N.lib()
+    assert L.d3h_problem_check(ctypes.byref(_problem(8, 9, 10))) == N.D3H_OK
+    bad = [((8, 9, 10), dict(coef=(0.2, 0.2, 0.2, 0.2, 0.2, 0.2, -0.2)), N.D3H_ERR_UNSTABLE),
+           ((8, 9, 10), dict(coef=(0.1,) * 6 + (0.5,)), N.D3H_ERR_UNSTABLE),
+           ((8, 9, 10), dict(coef=(float("nan"),) + (0.1,) * 5 + (0.5,)), N.D3H_ERR_UNSTABLE),
+           ((0, 9, 10), {}, N.D3H_ERR_INVALID),
+           ((8, 9, 10), dict(steps=-1), N.D3H_ERR_INVALID),
+           ((8, 9, 10), dict(dtype=7), N.D3H_ERR_INVALID),
+           ((8, 9, 10), dict(rounding=1), N.D3H_ERR_UNSUPPORTED),       # stochastic + FP32
+           ((8, 9, 10), dict(rounding=2, dtype=N.BF16), N.D3H_ERR_INVALID)]
+    for dims, kw, code in bad:
+        assert L.d3h_problem_check(ctypes.byref(_problem(*dims, **kw))) == code, (dims, kw)
+    assert L.d3h_problem_check(ctypes.byref(_problem(8, 9, 10, dtype=N.BF16, rounding=1))) == 0
+
+
+def test_request_footprint_on_the_host():
+    L = N.lib()
+
+    def fp(p, hd, od):
+        b = ctypes.c_size_t()
+        assert L.d3h_problem_footprint(ctypes.byref(p), hd, od, ctypes.byref(b)) == 0
+        return b.value
+
+    def up(v):
+        return (v + 255) // 256 * 256
+    # FP32, rows already 16-byte multiples: two buffers, no staging
+    assert fp(_problem(64, 10, 7), N.F32, N.F32) == 2 * up(64 * 10 * 7 * 4)
+    # FP32 with padded rows (nx = 30 -> pitch 32): staging of the dense host size
+    assert fp(_problem(30, 10, 7), N.F32, N.F32) == 2 * up(32 * 10 * 7 * 4) + up(30 * 70 * 4)
+    # BF16 storage from / to FP32 host arrays: FP32-sized staging
+    assert fp(_problem(64, 10, 7, dtype=N.BF16), N.F32, N.F32) == 2 * up(64 * 70 * 2) + up(64 * 70 * 4)
+    b = ctypes.c_size_t()
+    assert L.d3h_problem_footprint(ctypes.byref(_problem(64, 10, 7, coef=(1,) * 7)), N.F32, N.F32,
+                                   ctypes.byref(b)) == N.D3H_ERR_UNSTABLE
