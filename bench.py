@@ -437,7 +437,8 @@ def main():
         name = "stencil_kernel<%d, %d, %d, %d, %d, %d, %s, %d>" % (
             t["dtype"], vec, t["temporal_depth"], t["groups"], t["rows_per_wave"], t["waves"],
             "true" if t["arith"] else "false", t["prefetch"])
-        if prof.get("kernel") == name:
+        grid = tm["blocks"] * 64 * t["waves"]
+        if prof.get("kernel") == name and prof.get("grid_size", grid) == grid:
             traffic = prof.get("hbm_bytes_per_launch")
 
     cpu = None

@@ -308,6 +308,12 @@ struct TileRunner {
   static constexpr int TILE_X = G * GSTRIDE;
   static constexpr int TILE_Y = NW * RPW;
   static constexpr int NS = PF + 3;
+  // Tiles taller than their 2T halo rows have rows some stages can skip.
+  // A tile taller than its 2T halo rows has halo rows whose last-stage values
+  // are never stored; the last stage skips their arithmetic.  (Skipping the
+  // rows that earlier stages need not compute as well was measured: the extra
+  // branches cost the deep tiles registers and occupancy, so it is not done.)
+  static constexpr bool kSkipHaloRows = TILE_Y > 2 * T;
 
   // lv[level][slot][row][cell]: level 0 is the loaded input.
   C lv[T][NS][RPW][NV];
@@ -326,6 +332,7 @@ struct TileRunner {
   int ldoff[RPW][G];   // byte offset in a plane, or kOutOfRange (loads zeros)
   int stoff[RPW][G];   // byte offset in a plane, or kOutOfRange (store dropped)
   bool rowlast[RPW];
+  bool rowhalo[RPW];    // row is not stored (uniform per wave)
 
   __device__ __forceinline__ TileRunner(const ST *s_, ST *d_, const Geometry &g, const Coef<C> &k_,
                                         C *x_, int tx, int ty, int tz)
@@ -354,6 +361,7 @@ struct TileRunner {
       const bool rowin = y < geo.ny;
       const bool rowst = y >= yo0 && y < yo1;
       rowlast[r] = y == geo.ny - 1;
+      rowhalo[r] = __builtin_amdgcn_readfirstlane(rowst ? 0 : 1) != 0;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int xv = xlane + g * GSTRIDE;
@@ -423,6 +431,12 @@ struct TileRunner {
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const C *cv = lv[s - 1][sc][r];
+      if (s == T && kSkipHaloRows && rowhalo[r]) {
+        // uniform branch: the store of this row is dropped anyway
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[r][v] = cv[v];
+        continue;
+      }
       // rows above / below this wave's first / last row
       C nrow[NV], srow[NV];
       if (r == 0) {
@@ -490,6 +504,7 @@ struct TileRunner {
         const uint32_t step = (uint32_t)(geo.step0 + s - 1);
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
+          if (s == T && kSkipHaloRows && rowhalo[r]) continue;
           const uint32_t prow =
               ((uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r)) * ((uint32_t)(geo.nx + 1) >> 1);
           const uint32_t sbase = step * 0x85EBCA77u + geo.seed;
