@@ -308,11 +308,10 @@ struct TileRunner {
   static constexpr int TILE_X = G * GSTRIDE;
   static constexpr int TILE_Y = NW * RPW;
   static constexpr int NS = PF + 3;
-  // Tiles taller than their 2T halo rows have rows some stages can skip.
   // A tile taller than its 2T halo rows has halo rows whose last-stage values
-  // are never stored; the last stage skips their arithmetic.  (Skipping the
-  // rows that earlier stages need not compute as well was measured: the extra
-  // branches cost the deep tiles registers and occupancy, so it is not done.)
+  // are never stored; the last stage skips their arithmetic.  (Skipping, at
+  // stages 1..T-1, the rows those stages need not compute either was measured
+  // slower: its branches cost the deep tiles registers or scheduling freedom.)
   static constexpr bool kSkipHaloRows = TILE_Y > 2 * T;
 
   // lv[level][slot][row][cell]: level 0 is the loaded input.
