@@ -369,8 +369,9 @@ int d3h_server_create(d3h_context *ctx, const d3h_server_config *cfg_in, d3h_ser
   ok = ok && hipEventCreateWithFlags(&s->sweep_ev[0], hipEventDisableTiming) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&s->sweep_ev[1], hipEventDisableTiming) == hipSuccess;
   if (!ok) {
+    const size_t pool_bytes = s->pool_bytes;
     d3h_server_destroy(s);
-    return fail(D3H_ERR_NOMEM, "server resources (pool of %zu bytes)", s->pool_bytes);
+    return fail(D3H_ERR_NOMEM, "server resources (pool of %zu bytes)", pool_bytes);
   }
   s->alloc.reset(s->pool_bytes);
   s->th = std::thread(scheduler, s);
