@@ -283,6 +283,33 @@ def test_full_size_768_depths_equal_plain(ctx):
                 assert c["max_abs"] == 0.0, depth
 
 
+def test_field_above_4gib_bitwise(ctx):
+    """A 1024 x 1024 x 1040 FP32 field (4.36 GB per buffer, beyond 32-bit
+    byte offsets): 3 plain EXACT steps bitwise equal to the oracle, and 5
+    steps at depth 4 bitwise equal to 5 plain steps."""
+    shape = (1024, 1024, 1040)
+    k = d3.constants_f32(*shape)
+    f0 = d3.initialize_f32(*shape, k)
+    assert f0.nbytes > 2 ** 32
+    print("1024x1024x1040 oracle ...", flush=True)
+    want = oracle.run_f32(f0, k[8:15], 3)
+    with d3.Field(ctx, *shape) as plain:
+        plain.upload(f0)
+        plain.run(3, d3.coefficients(k))
+        out = plain.download()
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+        del out, want
+        plain.upload(f0)
+        plain.run(5, d3.coefficients(k), depth=1)
+        with d3.Field(ctx, *shape) as tb:
+            tb.upload(f0)
+            tm = tb.run(5, d3.coefficients(k), depth=4)
+            assert tm["steps"] == 5
+            c = tb.compare(plain)
+    print("1024x1024x1040 depth 4 vs plain:", c, flush=True)
+    assert c["max_abs"] == 0.0
+
+
 def test_fortran_shaped_by_reference_entry():
     """diffusion_hip_ in the Fortran variant's calling convention: f1 advanced
     by exactly `count` steps (odd counts too), bitwise as the reference's CPU
