@@ -311,8 +311,10 @@ struct TileRunner {
   // A tile taller than its 2T halo rows has halo rows whose last-stage values
   // are never stored; the last stage skips their arithmetic.  (Skipping, at
   // stages 1..T-1, the rows those stages need not compute either was measured
-  // slower: its branches cost the deep tiles registers or scheduling freedom.)
-  static constexpr bool kSkipHaloRows = TILE_Y > 2 * T;
+  // slower: its branches cost the deep tiles registers or scheduling freedom.
+  // FP64 tiles, whose 16 bytes per cell keep them far from the VALU limit,
+  // measured 1.4 % slower with the skip, so they compute every row.)
+  static constexpr bool kSkipHaloRows = TILE_Y > 2 * T && S != kF64;
 
   // lv[level][slot][row][cell]: level 0 is the loaded input.
   C lv[T][NS][RPW][NV];
