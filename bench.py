@@ -172,6 +172,15 @@ def bench_batched(args, d3, ctx, dname):
     with d3.Batch(ctx, probs[:16]) as b:
         b.upload_all(inits[:16], d3.F32)
         b.run(arith=arith, depth=depth, graph=True)
+    # batch autotuner: depth, tile and z-chunks per size class, chosen on this
+    # batch's own sweeps (a one-time cost; the table can be saved and reused)
+    tune = None
+    if args.batch_tune and depth == 0:
+        t_tune = time.perf_counter()
+        with d3.Batch(ctx, probs) as b:
+            tune = b.autotune(arith)
+        tune["seconds"] = round(time.perf_counter() - t_tune, 2)
+        print("batch autotune:", tune, file=sys.stderr, flush=True)
     # end to end: create, upload, run (graph capture included), download
     t0 = time.perf_counter()
     b = d3.Batch(ctx, probs)
@@ -207,8 +216,10 @@ def bench_batched(args, d3, ctx, dname):
         "scaling": "weak", "vs_baseline": None, "dtype": dname,
         "data": "synthetic (seeded dims log-uniform [32,256], cosine product + 1e-3 noise)",
         "bf16_rounding": args.bf16_rounding if dtype == d3.BF16 else None,
-        "config": {"workload": "%d grids %s, %d steps, %s, %s" % (n, dname.upper(), steps, depth_desc(depth),
-                                                                          args.arith),
+        "config": {"workload": "%d grids %s, %d steps, %s, %s" % (
+                       n, dname.upper(), steps,
+                       "depth, tile and z-chunks per size class (batch autotuner)" if tune
+                       else depth_desc(depth), args.arith),
                    "cells": cells, "host_buffers": "pinned (batch and per-request alike)",
                    "parallelism": "single GPU"},
         "device_resident_solves_per_s": round(n / t_dev, 2),
@@ -217,6 +228,7 @@ def bench_batched(args, d3, ctx, dname):
         "speedup_vs_per_request": round(t_base / t_e2e, 2),
         "results_equal_per_request": bool(same),
         "error_vs_fp64": err,
+        "batch_autotune": tune,
     }
 
 
@@ -318,6 +330,9 @@ def main():
                     help="temporal depth (default per workload); 0 = the autotuner's choice")
     ap.add_argument("--graph", type=int, default=1, help="replay a captured hipGraph (1) or launch")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile before timing")
+    ap.add_argument("--batch-tune", type=int, default=0,
+                    help="batched: run the batch autotuner (depth, tile, z-chunks per size class) "
+                         "before timing")
     ap.add_argument("--cpu-steps", type=int, default=500, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU oracle comparison")

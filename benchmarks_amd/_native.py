@@ -117,6 +117,7 @@ SIGNATURES = {
     "d3h_batch_upload_all": (c_int, [c_void_p, P(c_void_p), c_int]),
     "d3h_batch_download_all": (c_int, [c_void_p, P(c_void_p), c_int]),
     "d3h_batch_run": (c_int, [c_void_p, c_int, c_int, c_int, P(Timing)]),
+    "d3h_batch_autotune": (c_int, [c_void_p, c_int, P(c_int), P(c_double), P(c_double)]),
     "d3h_server_create": (c_int, [c_void_p, P(ServerConfig), P(c_void_p)]),
     "d3h_server_destroy": (c_int, [c_void_p]),
     "d3h_server_submit": (c_int, [c_void_p, P(Problem), c_void_p, c_int, c_void_p, c_int,

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "plan.hpp"
 
@@ -34,16 +35,20 @@ int check_problem(const d3h_problem &p) {
   return D3H_OK;
 }
 
-const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, int T, int fma) {
+const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, int T, int fma,
+                                    int *zchunks) {
+  if (zchunks) *zchunks = 0;
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz));
-    if (it != ctx->tuned.end()) {
-      for (const auto &e : registry())
-        if (e.launch_batch && e.dtype == dtype && e.T == T && e.fma == fma && e.vec == it->second.vec &&
-            e.G == it->second.G && e.RPW == it->second.RPW && e.NW == it->second.NW &&
-            e.pf == it->second.pf)
-          return &e;
+    for (int batch : {1, 0}) {
+      auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz, batch));
+      if (it == ctx->tuned.end()) continue;
+      const TuneVal &v = it->second;
+      const KernelEntry *e = find_entry(dtype, T, fma, v.vec, v.G, v.RPW, v.NW, v.pf);
+      if (e && e->launch_batch) {
+        if (zchunks && batch) *zchunks = v.zchunks;
+        return e;
+      }
     }
   }
   const KernelEntry *best = nullptr;
@@ -64,6 +69,15 @@ const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, in
   return best;
 }
 
+int batch_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, int dflt) {
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tuned.find(tune_key(dtype, 0, fma, nx, ny, nz, 1));
+    if (it != ctx->tuned.end()) return it->second.T;
+  }
+  return tuned_depth(ctx, dtype, fma, nx, ny, nz, dflt);
+}
+
 // Chunk depth for batched members: classes fill the device together, so a
 // member only needs enough z-chunks to keep its halo planes a small overhead.
 static int batch_zchunks(int nz, int T) { return std::max(1, nz / std::max(32, 8 * T)); }
@@ -71,18 +85,18 @@ static int batch_zchunks(int nz, int T) { return std::max(1, nz / std::max(32, 8
 int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int fma, SweepPlan *plan) {
   const int n = (int)ms.size();
   std::vector<const KernelEntry *> ea(n, nullptr), eb(n, nullptr);
-  std::vector<int> qa(n, 0), rb(n, 0), ta(n, 1);
+  std::vector<int> qa(n, 0), rb(n, 0), ta(n, 1), za(n, 0), zb(n, 0);
   for (int i = 0; i < n; ++i) {
     const PlanMember &m = ms[i];
-    // depth 0: the depth the autotuner chose for the member's size class, else
-    // 2 for FP32/FP64 storage and 1 for 16-bit storage (whose kernels are
+    // depth 0: the depth the autotuners chose for the member's size class,
+    // else 2 for FP32/FP64 storage and 1 for 16-bit storage (whose kernels are
     // VALU-bound, so fused steps' halo work costs more than the traffic saved)
     const int dflt = dtype_bytes(m.dtype) >= 4 ? 2 : 1;
-    int Tm = T != 0 ? T : tuned_depth(ctx, m.dtype, fma, m.nx, m.ny, m.nz, dflt);
-    const KernelEntry *e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, Tm, fma);
+    int Tm = T != 0 ? T : batch_depth(ctx, m.dtype, fma, m.nx, m.ny, m.nz, dflt);
+    const KernelEntry *e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, Tm, fma, &za[i]);
     if (!e && Tm != 1) {
       Tm = 1;
-      e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, 1, fma);
+      e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, 1, fma, &za[i]);
     }
     if (!e) return fail(D3H_ERR_UNSUPPORTED, "no batched kernel for dtype %d", m.dtype);
     ea[i] = e;
@@ -90,12 +104,12 @@ int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int f
     qa[i] = m.steps / Tm;
     rb[i] = m.steps % Tm;
     if (rb[i]) {
-      eb[i] = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, 1, fma);
+      eb[i] = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, 1, fma, &zb[i]);
       if (!eb[i]) return fail(D3H_ERR_UNSUPPORTED, "no batched plain kernel for dtype %d", m.dtype);
     }
   }
   auto make = [&](const std::vector<const KernelEntry *> &ent, const std::vector<int> &cnt,
-                  std::vector<ClassPlan> &out, bool tail) -> int {
+                  const std::vector<int> &zc, std::vector<ClassPlan> &out, bool tail) -> int {
     std::vector<const KernelEntry *> uniq;
     for (int i = 0; i < n; ++i)
       if (ent[i] && cnt[i] > 0 && std::find(uniq.begin(), uniq.end(), ent[i]) == uniq.end())
@@ -114,8 +128,9 @@ int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int f
         m.buf[0] = pm.buf[0];
         m.buf[1] = pm.buf[1];
         unsigned nb = 0;
-        D3H_TRY(make_geometry(*e, pm.nx, pm.ny, pm.nz, pm.pitch, batch_zchunks(pm.nz, e->T), ctx->num_cus,
-                              occupancy_of(*e), &m.geo, &nb));
+        const int nzc = zc[i] > 0 ? zc[i] : batch_zchunks(pm.nz, e->T);
+        D3H_TRY(make_geometry(*e, pm.nx, pm.ny, pm.nz, pm.pitch, nzc, ctx->num_cus, occupancy_of(*e),
+                              &m.geo, &nb));
         for (int k = 0; k < 7; ++k) m.coef[k] = pm.coef[k];
         m.block_start = blocks;
         m.parity0 = tail ? (pm.parity0 + qa[i]) & 1 : pm.parity0;
@@ -135,8 +150,8 @@ int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int f
   };
   plan->A.clear();
   plan->B.clear();
-  D3H_TRY(make(ea, qa, plan->A, false));
-  D3H_TRY(make(eb, rb, plan->B, true));
+  D3H_TRY(make(ea, qa, za, plan->A, false));
+  D3H_TRY(make(eb, rb, zb, plan->B, true));
   size_t total = 0;
   int launches = 0;
   for (auto *cs : {&plan->A, &plan->B}) {
@@ -380,6 +395,146 @@ int d3h_batch_download_all(d3h_batch *b, void *const *hosts, int host_dtype) {
                      static_cast<char *>(b->staging) + offs[i], st));
   }
   D3H_HIP(hipStreamSynchronize(st));
+  return D3H_OK;
+}
+
+// Batch autotuner.  The batch's problems are grouped by size class (dtype and
+// ceil log2 of each dimension).  Class by class, every candidate (depth 1, 2
+// or 4 with each batched tile of that depth, then z-chunk counts 1, 2, 4 and
+// 8 for the best of them) is timed as a sweep of kTuneSteps steps over the
+// WHOLE batch, the other classes keeping their current choice, because
+// classes share the device in a real run; the fastest stays.  The choices go
+// to the context's tune table as batch entries (depth 0 and the chosen depth),
+// which depth-0 batch runs and servers then use.
+static constexpr int kTuneSteps = 8;  // a multiple of every batched depth
+static constexpr int kTuneReps = 5;
+
+static int time_batch_sweep(d3h_batch *b, const std::vector<PlanMember> &ms, int fma, void **tables,
+                            size_t *cap, std::vector<char> *host, double *best_ms) {
+  d3h_context *ctx = b->ctx;
+  SweepPlan plan;
+  D3H_TRY(plan_sweep(ctx, ms, 0, fma, &plan));
+  if (plan.table_bytes > *cap) {
+    if (*tables) (void)hipFree(*tables);
+    *tables = nullptr;
+    *cap = 0;
+    if (hipMalloc(tables, plan.table_bytes) != hipSuccess) return fail(D3H_ERR_NOMEM, "tune tables");
+    *cap = plan.table_bytes;
+  }
+  host->resize(plan.table_bytes);
+  plan_fill(plan, host->data());
+  if (plan.table_bytes)
+    D3H_HIP(hipMemcpy(*tables, host->data(), plan.table_bytes, hipMemcpyHostToDevice));
+  int launches = 0;
+  *best_ms = 1e300;
+  for (int r = 0; r <= kTuneReps; ++r) {  // r == 0 warms up
+    D3H_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    D3H_TRY(plan_enqueue(plan.A, *tables, ctx->stream, b->streams, b->events, &launches));
+    D3H_TRY(plan_enqueue(plan.B, *tables, ctx->stream, b->streams, b->events, &launches));
+    D3H_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    D3H_HIP(hipEventSynchronize(ctx->ev1));
+    float t = 0;
+    D3H_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (r > 0) *best_ms = std::min(*best_ms, (double)t);
+  }
+  return D3H_OK;
+}
+
+int d3h_batch_autotune(d3h_batch *b, int arith, int *classes, double *ms_before, double *ms_after) {
+  if (!b) return fail(D3H_ERR_INVALID, "batch is NULL");
+  if (arith != D3H_ARITH_EXACT && arith != D3H_ARITH_FMA) return fail(D3H_ERR_INVALID, "arith %d", arith);
+  const int fma = arith == D3H_ARITH_FMA;
+  d3h_context *ctx = b->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  release_plan(b);  // the cached sequence may change with the choices
+  std::vector<PlanMember> ms;
+  std::map<TuneKey, std::vector<int>> groups;  // depth-0 batch key -> members
+  for (size_t i = 0; i < b->probs.size(); ++i) {
+    const d3h_problem &p = b->probs[i];
+    PlanMember m{};
+    m.nx = p.nx; m.ny = p.ny; m.nz = p.nz; m.dtype = p.dtype; m.pitch = b->pitch[i];
+    char *base = static_cast<char *>(b->pool) + b->off[i];
+    m.buf[0] = base;
+    m.buf[1] = base + b->bytes[i];
+    m.parity0 = b->cur[i];
+    m.steps = kTuneSteps;
+    for (int k = 0; k < 7; ++k) m.coef[k] = p.coef[k];
+    m.sr = p.rounding == D3H_ROUND_STOCHASTIC;
+    m.seed = p.seed;
+    ms.push_back(m);
+    groups[tune_key(p.dtype, 0, fma, p.nx, p.ny, p.nz, 1)].push_back((int)i);
+  }
+  void *tables = nullptr;
+  size_t cap = 0;
+  std::vector<char> host;
+  auto set_entry = [&](TuneKey k, const KernelEntry &e, int zc, double t) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tuned[k] = TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, zc, t, e.T};
+  };
+  auto with_depth = [](TuneKey k, int T) {
+    k.T = T;
+    return k;
+  };
+  int rc = D3H_OK;
+  double t0 = 0, cur = 0;
+  int tuned = 0;
+  rc = time_batch_sweep(b, ms, fma, &tables, &cap, &host, &t0);
+  cur = t0;
+  for (auto it = groups.begin(); rc == D3H_OK && it != groups.end(); ++it) {
+    const TuneKey &k0 = it->first;
+    const PlanMember &rep = ms[it->second[0]];
+    std::map<int, std::pair<const KernelEntry *, double>> best_of;  // depth -> fastest tile
+    const KernelEntry *best = nullptr;
+    int best_zc = 0;
+    double best_t = 1e300;
+    // candidates: depth and tile (the depth-0 entry selects the depth)
+    for (int T : {1, 2, 4}) {
+      for (const auto &e : registry()) {
+        if (!e.launch_batch || e.dtype != k0.dtype || e.T != T || e.fma != fma || occupancy_of(e) <= 0) continue;
+        set_entry(k0, e, 0, 0.0);
+        set_entry(with_depth(k0, T), e, 0, 0.0);
+        double t = 0;
+        rc = time_batch_sweep(b, ms, fma, &tables, &cap, &host, &t);
+        if (rc != D3H_OK) break;
+        auto bo = best_of.find(T);
+        if (bo == best_of.end() || t < bo->second.second) best_of[T] = {&e, t};
+        if (t < best_t) {
+          best_t = t;
+          best = &e;
+        }
+      }
+      if (rc != D3H_OK) break;
+    }
+    if (rc != D3H_OK) break;
+    if (!best) continue;  // no batched kernel for this dtype
+    // every tried depth keeps its fastest tile (explicit-depth batch runs)
+    for (const auto &kv : best_of)
+      set_entry(with_depth(k0, kv.first), *kv.second.first, 0, kv.second.second / kTuneSteps);
+    // then the z-chunk count of the best
+    set_entry(k0, *best, 0, 0.0);
+    for (int zc : {1, 2, 4, 8}) {
+      if (zc > rep.nz) continue;
+      set_entry(k0, *best, zc, 0.0);
+      set_entry(with_depth(k0, best->T), *best, zc, 0.0);
+      double t = 0;
+      rc = time_batch_sweep(b, ms, fma, &tables, &cap, &host, &t);
+      if (rc != D3H_OK) break;
+      if (t < best_t) {
+        best_t = t;
+        best_zc = zc;
+      }
+    }
+    if (rc != D3H_OK) break;
+    set_entry(k0, *best, best_zc, best_t / kTuneSteps);
+    set_entry(with_depth(k0, best->T), *best, best_zc, best_t / kTuneSteps);
+    cur = best_t;
+    ++tuned;
+  }
+  if (tables) (void)hipFree(tables);
+  if (rc != D3H_OK) return rc;
+  if (classes) *classes = tuned;
+  if (ms_before) *ms_before = t0;
+  if (ms_after) *ms_after = cur;
   return D3H_OK;
 }
 

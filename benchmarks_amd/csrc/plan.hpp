@@ -43,9 +43,15 @@ struct SweepPlan {
   int launches = 0;
 };
 
-// Kernel for one member: tuned entry of its size class if any, else the
-// narrowest batched tile covering nx.  Returns nullptr if none exists.
-const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, int T, int fma);
+// Kernel for one member at depth T: the batch autotuner's entry for its size
+// class, else the single-grid autotuner's entry if it has a batched kernel,
+// else the narrowest batched tile covering nx.  *zchunks (if given) gets the
+// batch autotuner's z-chunk count, or 0 for the default.  nullptr if none.
+const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, int T, int fma,
+                                    int *zchunks = nullptr);
+// Depth for a member run at depth 0: the batch autotuner's choice for its size
+// class, else the single-grid autotuner's, else `dflt`.
+int batch_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, int dflt);
 
 // Build the class plans (host side) and the table layout.
 int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &members, int T, int fma,

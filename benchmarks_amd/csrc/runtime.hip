@@ -125,11 +125,11 @@ static int ilog2ceil(int v) {
   return r;
 }
 
-TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz) {
-  return TuneKey{dtype, T, fma, ilog2ceil(nx), ilog2ceil(ny), ilog2ceil(nz)};
+TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz, int batch) {
+  return TuneKey{dtype, T, fma, ilog2ceil(nx), ilog2ceil(ny), ilog2ceil(nz), batch};
 }
 
-static const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf) {
+const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf) {
   for (const auto &e : registry())
     if (e.dtype == dtype && e.T == T && e.fma == fma && e.vec == vec && e.G == G && e.RPW == RPW &&
         e.NW == NW && e.pf == pf)
@@ -1037,14 +1037,15 @@ int d3h_autotune_save(d3h_context *ctx, const char *path) {
   std::ofstream os(path);
   if (!os) return fail(D3H_ERR_INVALID, "cannot write %s", path);
   os << "# dtype depth fma log2nx log2ny log2nz vec groups rows_per_wave waves prefetch zchunks "
-        "ms_per_step entry_depth   (depth 0 = the depth chosen for the size class)\n";
+        "ms_per_step entry_depth batch   (depth 0 = the depth chosen for the size class; "
+        "batch 1 = chosen for batched sweeps)\n";
   std::lock_guard<std::mutex> lk(ctx->mu);
   for (const auto &kv : ctx->tuned) {
     const auto &k = kv.first;
     const auto &v = kv.second;
     os << k.dtype << ' ' << k.T << ' ' << k.fma << ' ' << k.cx << ' ' << k.cy << ' ' << k.cz << ' '
        << v.vec << ' ' << v.G << ' ' << v.RPW << ' ' << v.NW << ' ' << v.pf << ' ' << v.zchunks << ' '
-       << v.ms_per_step << ' ' << v.T << '\n';
+       << v.ms_per_step << ' ' << v.T << ' ' << k.batch << '\n';
   }
   return os ? D3H_OK : fail(D3H_ERR_INVALID, "write to %s failed", path);
 }
@@ -1063,8 +1064,12 @@ int d3h_autotune_load(d3h_context *ctx, const char *path) {
     if (!(ss >> k.dtype >> k.T >> k.fma >> k.cx >> k.cy >> k.cz >> v.vec >> v.G >> v.RPW >> v.NW >>
           v.pf >> v.zchunks >> v.ms_per_step))
       return fail(D3H_ERR_INVALID, "malformed line in %s: %s", path, line.c_str());
-    // entry depth (absent in tables without depth-0 lines: the key's depth)
+    // entry depth (absent in tables without depth-0 lines: the key's depth),
+    // then the batch flag (absent in tables without batched choices: 0)
     if (!(ss >> v.T)) v.T = k.T;
+    else if (!(ss >> k.batch)) k.batch = 0;
+    if (k.batch != 0 && k.batch != 1)
+      return fail(D3H_ERR_INVALID, "bad batch flag in %s: %s", path, line.c_str());
     if (v.T <= 0 || (k.T != 0 && v.T != k.T))
       return fail(D3H_ERR_INVALID, "bad entry depth in %s: %s", path, line.c_str());
     // entries no longer compiled are skipped

@@ -49,8 +49,10 @@ int occupancy_of(const KernelEntry &e);
 
 struct TuneKey {
   int dtype, T, fma, cx, cy, cz;
+  int batch;  // 1: chosen for batched sweeps (d3h_batch_autotune), 0: single grids
   bool operator<(const TuneKey &o) const {
-    return std::tie(dtype, T, fma, cx, cy, cz) < std::tie(o.dtype, o.T, o.fma, o.cx, o.cy, o.cz);
+    return std::tie(dtype, T, fma, cx, cy, cz, batch) <
+           std::tie(o.dtype, o.T, o.fma, o.cx, o.cy, o.cz, o.batch);
   }
 };
 struct TuneVal {
@@ -61,7 +63,9 @@ struct TuneVal {
 };
 // Size class of a grid: ceil(log2) of each dimension.  Depth 0 keys hold the
 // depth the autotuner chose for the class (d3h_autotune with depth 0).
-TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz);
+TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz, int batch = 0);
+// The compiled entry with these parameters, or nullptr.
+const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf);
 // The tuned depth of a size class, or `dflt` when the class has none.
 int tuned_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, int dflt);
 

@@ -367,6 +367,17 @@ class Batch:
               "d3h_batch_run")
         return t.as_dict()
 
+    def autotune(self, arith=N.ARITH_EXACT):
+        """Choose depth, tile and z-chunks per size class for batched sweeps of
+        this batch (the context keeps them).  Overwrites the fields: call it
+        before uploading.  Returns the classes tuned and the 8-step whole-batch
+        sweep time before and after."""
+        n = ctypes.c_int()
+        before, after = ctypes.c_double(), ctypes.c_double()
+        check(lib().d3h_batch_autotune(self.handle, arith, ctypes.byref(n), ctypes.byref(before),
+                                       ctypes.byref(after)), "d3h_batch_autotune")
+        return {"classes": n.value, "sweep_ms_before": before.value, "sweep_ms_after": after.value}
+
 
 def stable_coefficients(nx, ny, nz, kappa=0.1, dtype=np.float32):
     """Coefficients of the batched/served problems: dt = 0.1 h_min^2 / kappa with

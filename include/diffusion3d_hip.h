@@ -276,6 +276,19 @@ int d3h_batch_download_all(d3h_batch *b, void *const *hosts, int host_dtype);
  * use_graph 1 = capture + replay. */
 int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph,
                   d3h_timing *timing);
+/* Batch autotuner (the reference's tile sweep, diffusion3d/build.py:6-44,
+ * redone at run time for batched sweeps): for each size class in the batch
+ * (dtype and ceil log2 of each dimension), times depths 1, 2 and 4 with every
+ * batched tile, then z-chunk counts 1, 2, 4 and 8 for the fastest, each as an
+ * 8-step sweep of the whole batch (the other classes keep their choice), and
+ * keeps the fastest in the context's tune table as batch entries: depth-0
+ * batch runs and servers on this context use them, d3h_autotune_save/load
+ * keep them.  It runs the kernels on the batch's buffers, so it leaves their
+ * contents undefined: call it before uploading.  classes, ms_before and
+ * ms_after (may be NULL) get the number of classes tuned and the 8-step sweep
+ * time before and after. */
+int d3h_batch_autotune(d3h_batch *b, int arith, int *classes, double *ms_before,
+                       double *ms_after);
 
 /* ---- serving ----------------------------------------------------------------------
  * A request server over one device: requests (grid + steps + initial field)

@@ -116,3 +116,33 @@ def test_whole_batch_transfers_equal_per_problem(ctx):
                 b.download_all(outs[:-1], d3.F32)
     finally:
         arena.close()
+
+
+def test_batch_autotune_choices_keep_results(tmp_path):
+    """The batch autotuner's per-size-class choices (depth, tile, z-chunks,
+    all dtypes) change the launches, not the results: depth-0 runs after
+    tuning equal the oracle bitwise; the saved table holds batch entries and
+    a context that loads it produces the same bits."""
+    probs, inits = make_problems(20, seed=77, lo=8, hi=90, max_steps=13)
+    want = [expected(p, f) for p, f in zip(probs, inits)]
+    path = str(tmp_path / "batch_tune.txt")
+    with d3.Context(0) as ctx:
+        with d3.Batch(ctx, probs) as b:
+            info = b.autotune()
+            print("batch autotune:", info, flush=True)
+            assert info["classes"] >= 1
+            assert info["sweep_ms_after"] > 0
+            b.upload_all(inits, d3.F32)
+            b.run(depth=0, graph=True)
+            for i, p in enumerate(probs):
+                assert np.array_equal(bits(b.download(i)), bits(want[i])), (i, p)
+        ctx.save_tuning(path)
+    rows = [l.split() for l in open(path) if not l.startswith("#")]
+    assert any(r[-1] == "1" for r in rows)
+    with d3.Context(0) as ctx2:
+        ctx2.load_tuning(path)
+        with d3.Batch(ctx2, probs) as b:
+            b.upload_all(inits, d3.F32)
+            b.run(depth=0, graph=False)
+            for i, p in enumerate(probs):
+                assert np.array_equal(bits(b.download(i)), bits(want[i])), (i, p)
