@@ -368,3 +368,38 @@ def test_field_save_load_and_driver_raw_dump(ctx, tmp_path):
     k3 = d3.constants_f32(64, 64, 64)
     want = oracle.run_f32(d3.initialize_f32(64, 64, 64, k3), k3[8:15], 20)
     assert info["steps"] == 20 and np.array_equal(out.view(np.uint32), want.view(np.uint32))
+
+
+def test_contexts_on_concurrent_host_threads():
+    """One context per host thread: four threads each run their own field
+    (plain and depth 2, with and without graphs) at the same time; every
+    result is bitwise the oracle's."""
+    import threading
+    shapes = [(64, 48, 40), (50, 70, 33), (130, 9, 17), (33, 33, 65)]
+    results, errors = {}, []
+
+    def work(i, shape):
+        try:
+            k = d3.constants_f32(*shape)
+            f0 = d3.add_noise_f32(d3.initialize_f32(*shape, k), 900 + i, 1e-3)
+            with d3.Context(0) as c, d3.Field(c, *shape) as f:
+                outs = []
+                for depth, graph in ((1, False), (2, True)):
+                    f.upload(f0)
+                    f.run(21, d3.coefficients(k), depth=depth, graph=graph)
+                    outs.append(f.download())
+            results[i] = (f0, k, outs)
+        except Exception as e:  # reported by the main thread
+            errors.append((i, repr(e)))
+
+    threads = [threading.Thread(target=work, args=(i, s)) for i, s in enumerate(shapes)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(120)
+    assert not errors, errors
+    assert sorted(results) == list(range(len(shapes)))
+    for i, (f0, k, outs) in results.items():
+        want = oracle.run_f32(f0, k[8:15], 21)
+        for out in outs:
+            assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), i
