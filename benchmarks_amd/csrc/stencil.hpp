@@ -9,18 +9,21 @@
 //     TILE_Y = NW waves x RPW rows tall.  Lane l of group g holds cells
 //     x0 + g*64*VEC + l*VEC + [0, VEC) of each of its wave's RPW consecutive
 //     rows, so every HBM access is a coalesced 128-bit (or 64-bit) access.
-//   * The workgroup streams z: each iteration loads one plane (one iteration
-//     ahead of use) into registers.  Stage s (1..T) then computes time level s
-//     at plane i - s from level s-1 planes (i-s-1, i-s, i-s+1), all held in
-//     registers; plane q of a level lives in register slot q mod 4, and the
-//     z-loop is unrolled by 4 so slots are compile-time and never moved.
+//   * The workgroup streams z: iteration i requests plane i + PF (PF = 1..3
+//     planes in flight) into registers.  Stage s (1..T) then computes time
+//     level s at plane i - s from level s-1 planes (i-s-1, i-s, i-s+1), all
+//     held in registers; plane q of a level lives in register slot q mod NS
+//     (NS = PF + 3), and the z-loop is unrolled by NS so slots are
+//     compile-time and never moved.
 //   * x neighbours: inside a lane from registers, across lanes by DPP
 //     wave_shr:1 / wave_shl:1 (group seams via readlane).  y neighbours: from
 //     the wave's own rows in registers; a wave's first/last rows are exchanged
-//     with the adjacent waves through LDS (double-buffered, one barrier per
-//     plane).  Tile edges use the cell itself, which is exactly the clamped
-//     boundary at a domain edge; at interior tile edges the resulting invalid
-//     cells are never stored (tiles overlap by the halo, see Geometry).
+//     with the adjacent waves through LDS (double-buffered with one barrier
+//     per plane when that fits in 64 KB, else two barriers).  Tile edges use
+//     the cell itself, which is exactly the clamped boundary at a domain
+//     edge; at interior tile edges the resulting invalid cells are never
+//     stored (tiles overlap by the halo, see Geometry), and the last stage
+//     skips the arithmetic of the halo rows (their stores are dropped).
 //   * Clamp selects (x = nx-1, y = ny-1, z = 0 / nz-1) are only evaluated in
 //     the EDGE variant of a stage, used for tiles that contain the grid's last
 //     column or row beyond their own last cell, and for the first/last plane.
@@ -77,7 +80,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, in
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, bytes, kRsrcWord3);
 }
 
-// Launch geometry, computed on the host (make_geometry in runtime.cpp).
+// Launch geometry, computed on the host (make_geometry in runtime.hip).
 //   Tiles overlap: tile (tx,ty,tz) stores x in [tx*ox, tx*ox+ox), rows
 //   [ty*oy, ty*oy+oy), planes [tz*oz, tz*oz+oz) (clipped to the grid) and loads
 //   a window that extends hx cells / T rows / T planes beyond that on every
