@@ -27,7 +27,9 @@ cpu_baseline: the reference's OpenMP RunKernel (oracle/_ref/libref.so, built
 from the reference sources) or, when that library is absent, the oracle's C
 restatement, timed on this host's cores on a bounded sample of the same grid.
 
-Other workloads (--workload): plain512_f64, tb768_f32 (temporal blocking),
+Other workloads (--workload): plain512_f64, plain512_bf16 (BF16 storage, FP32
+arithmetic, stochastic rounding unless --bf16-rounding nearest; its CPU
+baseline is the FP32 reference), tb768_f32 (temporal blocking),
 plain16384_2d / tb16384_2d (the diffusion/ family's 2D 5-point operator on a
 16384^2 FP32 grid, row-streaming kernels, depth 1 / 8),
 batched512_f32 / batched512_bf16 (512 uneven grids per run, solves/s against
@@ -53,12 +55,15 @@ WORKLOADS = {
     # name: (nx, ny, nz, dtype, depth, default steps, seeded noise amplitude)
     "plain512_f32": (512, 512, 512, "f32", 1, 1000, 0.0),
     "plain512_f64": (512, 512, 512, "f64", 1, 1000, 0.0),
+    # BF16 storage, FP32 arithmetic (rounding per --bf16-rounding)
+    "plain512_bf16": (512, 512, 512, "bf16", 1, 1000, 0.0),
     "tb768_f32": (768, 768, 768, "f32", 4, 2000, 1e-3),
     # 2D 5-point operator (nz = 1): diffusion/diffusion_openmp.cc:59-82
     "plain16384_2d": (16384, 16384, 1, "f32", 1, 1000, 0.0),
     "tb16384_2d": (16384, 16384, 1, "f32", 8, 2000, 0.0),
 }
 NOISE_SEED = 768
+BF16_SEED = 0x5EED
 # cell updates of the CPU baseline sample per --cpu-steps step (the 512^3 grid)
 CPU_SAMPLE_CELLS = 512 ** 3
 
@@ -360,9 +365,14 @@ def main():
     nx, ny, nz, dname, depth, dsteps, noise = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else dsteps
     depth = args.depth if args.depth is not None else depth
-    dtype = {"f32": d3.F32, "f64": d3.F64}[dname]
+    dtype = {"f32": d3.F32, "f64": d3.F64, "bf16": d3.BF16}[dname]
     arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
-    esize = 4 if dtype == d3.F32 else 8
+    esize = {d3.F32: 4, d3.F64: 8, d3.BF16: 2}[dtype]
+    # BF16 storage rounds every stored level (stochastically with a fixed seed
+    # unless --bf16-rounding nearest)
+    rnd = {}
+    if dtype == d3.BF16 and args.bf16_rounding == "stochastic":
+        rnd = {"rounding": d3.ROUND_STOCHASTIC, "seed": BF16_SEED}
 
     ctx = d3.Context(0)
     info = ctx.info()
@@ -383,7 +393,7 @@ def main():
 
         def initial():
             return d3.initialize2d_f32(nx, ny, k)
-    elif dtype == d3.F32:
+    elif dtype in (d3.F32, d3.BF16):
         k = d3.constants_f32(nx, ny, nz)
 
         def initial():
@@ -400,15 +410,15 @@ def main():
         field.set_y_reversed(True)
     field.upload(initial())
     if args.warmup > 0:
-        field.run(args.warmup, coef, depth=depth, arith=arith, graph=False)
+        field.run(args.warmup, coef, depth=depth, arith=arith, graph=False, **rnd)
     # re-initialise after the warm-up (diffusion/diffusion.h:126-131)
     field.upload(initial())
     if args.graph:
         # capture once (untimed), restore the input, then time a replay
-        field.run(steps, coef, depth=depth, arith=arith, graph=True)
+        field.run(steps, coef, depth=depth, arith=arith, graph=True, **rnd)
         field.upload(initial())
     t0 = time.perf_counter()
-    tm = field.run(steps, coef, depth=depth, arith=arith, graph=bool(args.graph))
+    tm = field.run(steps, coef, depth=depth, arith=arith, graph=bool(args.graph), **rnd)
     wall = time.perf_counter() - t0
     print("timing:", tm, file=sys.stderr, flush=True)
 
@@ -423,7 +433,7 @@ def main():
     # the FP64 GPU oracle: the same initial field and coefficients advanced in
     # FP64 (EXACT arithmetic; fused depths are bitwise equal to plain steps)
     err64 = None
-    if dtype == d3.F32 and not args.no_fp64:
+    if dtype in (d3.F32, d3.BF16) and not args.no_fp64:
         t64 = time.perf_counter()
         with d3.Field(ctx, nx, ny, nz, d3.F64) as ref:
             if two_d:
@@ -465,6 +475,8 @@ def main():
 
     depth_used = tm["temporal_depth"]
     size = ("%d^2 (2D 5-point)" % nx) if two_d else ("%d^3" % nx)
+    if dtype == d3.BF16:
+        size += " (BF16 storage)"
     metric = "Gcell-updates/s + %%HBM roofline at %s" % size
     if depth_used > 1:
         metric = "effective Gcell-updates/s + %%HBM roofline at %s, temporal depth %d" % (size,
@@ -480,7 +492,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": dname,
+        "dtype": "bf16 storage, f32 compute" if dtype == d3.BF16 else dname,
+        "bf16_rounding": args.bf16_rounding if dtype == d3.BF16 else None,
         "data": "synthetic (analytical cosine-product field%s, reference coefficients)"
                 % (", + seeded uniform noise amplitude %g" % noise if noise else ""),
         "config": {"workload": "%dx%dx%d %s %s diffusion, %d steps, depth %d, %s arithmetic"
