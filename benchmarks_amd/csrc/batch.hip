@@ -483,6 +483,24 @@ int d3h_batch_autotune(d3h_batch *b, int arith, int *classes, double *ms_before,
   for (auto it = groups.begin(); rc == D3H_OK && it != groups.end(); ++it) {
     const TuneKey &k0 = it->first;
     const PlanMember &rep = ms[it->second[0]];
+    // the class's entries before this search, restored if it fails
+    std::map<TuneKey, TuneVal> saved;
+    std::vector<TuneKey> keys = {k0, with_depth(k0, 1), with_depth(k0, 2), with_depth(k0, 4)};
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      for (const auto &k : keys) {
+        auto f = ctx->tuned.find(k);
+        if (f != ctx->tuned.end()) saved[k] = f->second;
+      }
+    }
+    auto restore = [&] {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      for (const auto &k : keys) {
+        auto f = saved.find(k);
+        if (f != saved.end()) ctx->tuned[k] = f->second;
+        else ctx->tuned.erase(k);
+      }
+    };
     std::map<int, std::pair<const KernelEntry *, double>> best_of;  // depth -> fastest tile
     const KernelEntry *best = nullptr;
     int best_zc = 0;
@@ -505,8 +523,11 @@ int d3h_batch_autotune(d3h_batch *b, int arith, int *classes, double *ms_before,
       }
       if (rc != D3H_OK) break;
     }
-    if (rc != D3H_OK) break;
-    if (!best) continue;  // no batched kernel for this dtype
+    if (rc != D3H_OK || !best) {  // an error, or no batched kernel for this dtype
+      restore();
+      if (rc != D3H_OK) break;
+      continue;
+    }
     // every tried depth keeps its fastest tile (explicit-depth batch runs)
     for (const auto &kv : best_of)
       set_entry(with_depth(k0, kv.first), *kv.second.first, 0, kv.second.second / kTuneSteps);
@@ -524,7 +545,10 @@ int d3h_batch_autotune(d3h_batch *b, int arith, int *classes, double *ms_before,
         best_zc = zc;
       }
     }
-    if (rc != D3H_OK) break;
+    if (rc != D3H_OK) {
+      restore();
+      break;
+    }
     set_entry(k0, *best, best_zc, best_t / kTuneSteps);
     set_entry(with_depth(k0, best->T), *best, best_zc, best_t / kTuneSteps);
     cur = best_t;
