@@ -31,7 +31,7 @@ Other workloads (--workload): plain512_f64, plain512_bf16 (BF16 storage, FP32
 arithmetic, stochastic rounding unless --bf16-rounding nearest; its CPU
 baseline is the FP32 reference), tb768_f32 (temporal blocking),
 plain16384_2d / tb16384_2d (the diffusion/ family's 2D 5-point operator on a
-16384^2 FP32 grid, row-streaming kernels, depth 1 / 8),
+16384^2 FP32 grid, 2D kernels, depth 1 / 8),
 batched512_f32 / batched512_bf16 (512 uneven grids per run, solves/s against
 launching the plain kernel per request) and served (request server: solves/s,
 p50/p99 latency under Poisson arrivals).  Each prints its own JSON line.
@@ -459,9 +459,15 @@ def main():
             prof = json.load(fh)
         t = d3.tiles()[tm["tile"]]
         vec = t["tile_x"] // (64 * t["groups"])
-        name = "stencil_kernel<%d, %d, %d, %d, %d, %d, %s, %d>" % (
-            t["dtype"], vec, t["temporal_depth"], t["groups"], t["rows_per_wave"], t["waves"],
-            "true" if t["arith"] else "false", t["prefetch"])
+        fma = "true" if t["arith"] else "false"
+        if t["row_stream"]:
+            name = "stencil2d_kernel<%d, %d, %d, %d, %d, %s, %d>" % (
+                t["dtype"], vec, t["temporal_depth"], t["groups"], t["rows_per_wave"], fma,
+                t["prefetch"])
+        else:
+            name = "stencil_kernel<%d, %d, %d, %d, %d, %d, %s, %d>" % (
+                t["dtype"], vec, t["temporal_depth"], t["groups"], t["rows_per_wave"], t["waves"],
+                fma, t["prefetch"])
         grid = tm["blocks"] * 64 * t["waves"]
         if prof.get("kernel") == name and prof.get("grid_size", grid) == grid:
             traffic = prof.get("hbm_bytes_per_launch")

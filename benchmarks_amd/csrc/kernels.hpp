@@ -26,8 +26,9 @@ struct KernelEntry {
   LaunchFn launch;
   const void *batch_func;      // nullptr when no batched instantiation exists
   LaunchBatchFn launch_batch;
-  // 1: row-streaming entry (one wave, one row per streamed "plane"); it runs
-  // nz == 1 fields as a stream over their rows and nothing else.
+  // 1: 2D entry (stencil2d_kernel: one wave, RPW rows per streamed "plane",
+  // the 5-point update); it runs nz == 1 fields with ct == cb == 0 as a
+  // stream over their rows and nothing else.
   int rows = 0;
   int tile_x() const { return 64 * G * vec; }
   int tile_y() const { return NW * RPW; }
@@ -50,6 +51,16 @@ void launch_stencil_batch(const BatchMember *members, const int *block_member, i
                      0, stream, members, block_member, launch);
 }
 
+template <int S, int VEC, int T, int G, int RPW, bool FMA, int PF>
+void launch_stencil2d(const void *src, void *dst, const Geometry &g, const double *c, unsigned blocks,
+                      hipStream_t stream) {
+  using C = typename Traits<S>::C;
+  using ST = typename Traits<S>::T;
+  Coef<C> k{(C)c[0], (C)c[1], (C)c[2], (C)c[3], (C)c[4], (C)c[5], (C)c[6]};
+  hipLaunchKernelGGL((stencil2d_kernel<S, VEC, T, G, RPW, FMA, PF>), dim3(blocks), dim3(64), 0, stream,
+                     (const ST *)src, (ST *)dst, g, k);
+}
+
 // Single-grid kernel only (prefetch distance PF).
 #define D3H_ENTRY_PF(S, VEC, T, G, RPW, NW, FMA, PF)                                          \
   KernelEntry {                                                                               \
@@ -66,11 +77,11 @@ void launch_stencil_batch(const BatchMember *members, const int *block_member, i
         &launch_stencil_batch<S, VEC, T, G, RPW, NW, FMA, 1>                                  \
   }
 
-// Row-streaming kernel (RPW = NW = 1).
-#define D3H_ENTRY_ROW(S, VEC, T, G, FMA, PF)                                                  \
+// 2D kernel (one wave, RPW rows per streamed plane, 5-point update).
+#define D3H_ENTRY_2D(S, VEC, T, G, RPW, FMA, PF)                                              \
   KernelEntry {                                                                               \
-    S, VEC, T, G, 1, 1, FMA, PF, (const void *)&stencil_kernel<S, VEC, T, G, 1, 1, FMA, PF>,  \
-        &launch_stencil<S, VEC, T, G, 1, 1, FMA, PF>, nullptr, nullptr, 1                     \
+    S, VEC, T, G, RPW, 1, FMA, PF, (const void *)&stencil2d_kernel<S, VEC, T, G, RPW, FMA, PF>, \
+        &launch_stencil2d<S, VEC, T, G, RPW, FMA, PF>, nullptr, nullptr, 1                    \
   }
 
 // Each kernels_*.hip file defines one table.

@@ -141,44 +141,37 @@ bool rows_ok(const d3h_field *f, const double coef[7]) {
   return f->nz == 1 && coef[4] == 0.0 && coef[5] == 0.0;
 }
 
-// Row streaming.  An nx x ny x 1 field is run as a stream over its rows: the
-// tile runner sees one-row planes (ny' = 1, nz' = ny) walked from the last
-// row to the first (plane stride -pitch), so its z-1 / z+1 neighbours are
-// rows y+1 / y-1.  With coefficients (ce, cw, 0, 0, cn, cs, cc) in the
-// (ce cw cn cs ct cb cc) slots it evaluates
-//   cc c + cw w + ce e + 0 c + 0 c + cs f[y+1] + cn f[y-1],
-// the 7-point step of the nz == 1 field (whose clamped z terms are 0 c when
-// ct == cb == 0) with the y terms in the same order; adding the zero terms
-// before instead of after the y terms changes at most the sign of an exactly
-// zero result.
+// 2D kernels.  An nx x ny x 1 field with ct == cb == 0 is run as a stream over
+// its rows in blocks of RPW rows: the tile runner sees planes of RPW rows
+// (ny' = RPW, nz' = ceil(ny / RPW), plane stride RPW * pitch, the last block
+// clipped to the field by rows2d) and evaluates the 5-point update
+//   cc c + cw w + ce e + cs f[y+1] + cn f[y-1]
+// with y neighbours taken across block boundaries from the adjacent blocks.
+// This is the 7-point step of the nz == 1 field without its two clamped z
+// terms (0 c each when ct == cb == 0), which can differ from it at most in
+// the sign of an exactly zero result.
 int field_geometry(const KernelEntry &e, const d3h_field *f, int zchunks, Geometry *g,
                    unsigned *blocks) {
-  if (!e.rows)
-    return make_geometry(e, f->nx, f->ny, f->nz, f->pitch, zchunks, f->ctx->num_cus, occupancy_of(e), g,
-                         blocks);
-  if (f->nz != 1) return fail(D3H_ERR_INVALID, "row-streaming tile on a field with nz = %d", f->nz);
   const int occ = occupancy_of(e), cus = f->ctx->num_cus;
+  if (!e.rows) return make_geometry(e, f->nx, f->ny, f->nz, f->pitch, zchunks, cus, occ, g, blocks);
+  if (f->nz != 1) return fail(D3H_ERR_INVALID, "2D tile on a field with nz = %d", f->nz);
+  const int planes = (f->ny + e.RPW - 1) / e.RPW;
   if (zchunks <= 0) {
     // one full wave of resident workgroups, chunks of at least max(16T, 64)
-    // rows (a row is cheap, so the 2T halo rows must stay a small share)
-    D3H_TRY(make_geometry(e, f->nx, 1, f->ny, f->pitch, 1, cus, occ, g, blocks));
+    // rows (a row is cheap, so the T halo blocks must stay a small share)
+    D3H_TRY(make_geometry(e, f->nx, e.RPW, planes, f->pitch, 1, cus, occ, g, blocks));
     const long long slots = (long long)std::max(occ, 1) * std::max(cus, 1);
-    zchunks = (int)std::max(1LL, std::min(slots / g->ntx, (long long)(f->ny / std::max(16 * e.T, 64))));
+    const long long min_planes = (std::max(16 * e.T, 64) + e.RPW - 1) / e.RPW;
+    zchunks = (int)std::max(1LL, std::min(slots / g->ntx, (long long)planes / min_planes));
   }
-  D3H_TRY(make_geometry(e, f->nx, 1, f->ny, f->pitch, zchunks, cus, occ, g, blocks));
-  g->plane = -(long long)f->pitch;
+  D3H_TRY(make_geometry(e, f->nx, e.RPW, planes, f->pitch, zchunks, cus, occ, g, blocks));
+  g->rows2d = f->ny;
   return D3H_OK;
 }
 
-static void launch_entry(const KernelEntry &e, const d3h_field *f, const void *src, void *dst,
-                         const Geometry &g, const double coef[7], unsigned blocks, hipStream_t stream) {
-  if (!e.rows) {
-    e.launch(src, dst, g, coef, blocks, stream);
-    return;
-  }
-  const size_t last = (size_t)(f->ny - 1) * f->pitch * dtype_bytes(f->dtype);
-  const double rc[7] = {coef[0], coef[1], 0.0, 0.0, coef[2], coef[3], coef[6]};
-  e.launch(static_cast<const char *>(src) + last, static_cast<char *>(dst) + last, g, rc, blocks, stream);
+static void launch_entry(const KernelEntry &e, const void *src, void *dst, const Geometry &g,
+                         const double coef[7], unsigned blocks, hipStream_t stream) {
+  e.launch(src, dst, g, coef, blocks, stream);
 }
 
 int tuned_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, int dflt) {
@@ -251,7 +244,7 @@ int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEnt
   Geometry g{}, g1{};
   unsigned nb = 0, nb1 = 0;
   if (e.rows && !rows_ok(f, coef))
-    return fail(D3H_ERR_INVALID, "row-streaming tile needs nz == 1 and ct == cb == 0");
+    return fail(D3H_ERR_INVALID, "2D tile needs nz == 1 and ct == cb == 0");
   D3H_TRY(field_geometry(e, f, zchunks, &g, &nb));
   const KernelEntry *plain = nullptr;
   if (rem) {
@@ -267,12 +260,12 @@ int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEnt
   int n = 0;
   for (int i = 0; i < nfull; ++i, ++n) {
     g.step0 = (int)(base + (long long)i * T);
-    launch_entry(e, f, f->buf[f->cur], f->buf[f->cur ^ 1], g, coef, nb, stream);
+    launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], g, coef, nb, stream);
     f->cur ^= 1;
   }
   for (int i = 0; i < rem; ++i, ++n) {
     g1.step0 = (int)(base + (long long)nfull * T + i);
-    launch_entry(*plain, f, f->buf[f->cur], f->buf[f->cur ^ 1], g1, coef, nb1, stream);
+    launch_entry(*plain, f->buf[f->cur], f->buf[f->cur ^ 1], g1, coef, nb1, stream);
     f->cur ^= 1;
   }
   D3H_HIP(hipGetLastError());
@@ -947,7 +940,8 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
     Geometry g{};
     unsigned nb = 0;
     if (field_geometry(e, f, 0, &g, &nb) != D3H_OK) continue;
-    const int nstream = e.rows ? f->ny : f->nz;  // planes (rows) streamed
+    // planes (rows, row blocks) streamed
+    const int nstream = e.rows ? (f->ny + e.RPW - 1) / e.RPW : f->nz;
     // few long streams (1-4 chunks) up to several waves of workgroups
     std::vector<int> cands = {1, 2, 3, 4, g.nzc, g.nzc * 2, g.nzc * 4, std::max(1, g.nzc / 2)};
     std::sort(cands.begin(), cands.end());
@@ -957,11 +951,11 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
       Geometry g2{};
       unsigned nb2 = 0;
       if (field_geometry(e, f, zc, &g2, &nb2) != D3H_OK) continue;
-      launch_entry(e, f, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);  // warm-up
+      launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);  // warm-up
       f->cur ^= 1;
       (void)hipEventRecord(ctx->ev0, ctx->stream);
       for (int r = 0; r < reps; ++r) {
-        launch_entry(e, f, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);
+        launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);
         f->cur ^= 1;
       }
       (void)hipEventRecord(ctx->ev1, ctx->stream);
@@ -990,7 +984,7 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
   d3h_field *f = nullptr;
   D3H_TRY(d3h_field_create(ctx, nx, ny, nz, dtype, &f));
   // nz == 1 fields are tuned for the 2D operator (ct = cb = 0), which also
-  // admits the row-streaming tiles
+  // admits the 2D tiles
   const bool rows = nz == 1;
   const double coef[7] = {0.1, 0.1, 0.1, 0.1, rows ? 0.0 : 0.1, rows ? 0.0 : 0.1, rows ? 0.6 : 0.4};
   const auto &reg = registry();

@@ -38,7 +38,7 @@ int pitch_elems(int nx, int dtype);
 // All compiled kernels.
 const std::vector<KernelEntry> &registry();
 // Default entry for (dtype, T, fma) (first in table order), or nullptr.
-// rows selects the row-streaming entries instead of the 3D tiles.
+// rows selects the 2D entries (stencil2d_kernel) instead of the 3D tiles.
 const KernelEntry *default_entry(int dtype, int T, int fma, bool rows = false);
 
 // Geometry and block count of one launch of `e` over a grid; zchunks 0 =
@@ -112,15 +112,15 @@ struct d3h_field {
 };
 
 namespace d3h {
-// True when a field and coefficients can run on the row-streaming entries:
-// nz == 1 and ct == cb == 0 (the 2D 5-point operator).
+// True when a field and coefficients can run on the 2D entries: nz == 1 and
+// ct == cb == 0 (the 2D 5-point operator).
 bool rows_ok(const d3h_field *f, const double coef[7]);
 // Resolve the kernel for a run configuration (tuned / explicit / default);
-// row-streaming entries are chosen only when `rows` (see rows_ok).
+// 2D entries are chosen only when `rows` (see rows_ok).
 int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
                   bool rows, const KernelEntry **entry, int *zchunks);
-// Launch geometry of entry `e` on field `f` (row-streaming entries stream the
-// rows of the nz == 1 field).
+// Launch geometry of entry `e` on field `f` (2D entries stream the rows of
+// the nz == 1 field in blocks of RPW rows).
 int field_geometry(const KernelEntry &e, const d3h_field *f, int zchunks, Geometry *g,
                    unsigned *blocks);
 // Enqueue `count` steps from f->buf[f->cur] on `stream` (no sync); updates cur.

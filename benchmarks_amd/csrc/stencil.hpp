@@ -85,8 +85,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, in
 //   [ty*oy, ty*oy+oy), planes [tz*oz, tz*oz+oz) (clipped to the grid) and loads
 //   a window that extends hx cells / T rows / T planes beyond that on every
 //   side that is not a domain boundary.  pitch is the padded row length
-//   (multiple of 16 bytes); plane = pitch * ny, or -pitch when the rows of an
-//   nz == 1 field are streamed as planes from the last row to the first.
+//   (multiple of 16 bytes); plane = pitch * ny.
 struct Geometry {
   int nx, ny, nz, pitch;
   long long plane;
@@ -98,6 +97,10 @@ struct Geometry {
   int sr;
   unsigned seed;
   int step0;
+  // 2D fields streamed RPW rows per "plane" (stencil2d_kernel): the field's
+  // row count; plane q holds rows [q*RPW, q*RPW + RPW) (the last one clipped).
+  // 0 for every other kernel.
+  int rows2d;
 };
 
 template <typename C> struct Coef { C ce, cw, cn, cs, ct, cb, cc; };
@@ -281,6 +284,21 @@ __device__ __forceinline__ C update(const Coef<C> &k, C c, C w, C e, C s, C n, C
   }
 }
 
+// The 2D 5-point update, the diffusion/ family's term order
+// (diffusion/diffusion_openmp.cc:59-82 on a row-reversed field: s = device row
+// y + 1 = host row y - 1 first).
+template <typename C, bool FMA>
+__device__ __forceinline__ C update5(const Coef<C> &k, C c, C w, C e, C s, C n) {
+  if constexpr (FMA) {
+    C r = fma_(k.cc, c, k.cw * w);
+    r = fma_(k.ce, e, r);
+    r = fma_(k.cs, s, r);
+    return fma_(k.cn, n, r);
+  } else {
+    return k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n;
+  }
+}
+
 template <int N>
 constexpr int modn(int v) { return ((v % N) + N) % N; }
 
@@ -300,8 +318,15 @@ struct XchLayout {
 // PF = prefetch distance in planes: plane i + PF is requested while plane i
 // is computed.  Each level keeps NS = PF + 3 register slots (plane q in slot
 // q mod NS) and the z loop is unrolled by NS.
-template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF>
+//
+// W2D (one wave, NW = 1): the 2D 5-point operator on an nx x ny field whose
+// "planes" are blocks of RPW consecutive rows.  Row r's y neighbours are rows
+// r +- 1 of the same plane, or, across a plane boundary, the last row of the
+// previous plane / the first row of the next one (register slots of the
+// adjacent planes); there are no z terms.
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF, bool W2D = false>
 struct TileRunner {
+  static_assert(!W2D || NW == 1, "2D row blocks are streamed by one wave");
   using ST = typename Traits<S>::T;
   using C = typename Traits<S>::C;
   static constexpr int NV = G * VEC;
@@ -333,6 +358,8 @@ struct TileRunner {
   int zl0, zl1, zo0, zo1;
   bool edge_xy;         // tile holds the last column/row strictly inside it
   int plane_bytes;
+  int row_bytes;         // W2D: bytes of one row (pitch)
+  int lastr;             // W2D: row of the last plane holding the field's last row
   int ldoff[RPW][G];   // byte offset in a plane, or kOutOfRange (loads zeros)
   int stoff[RPW][G];   // byte offset in a plane, or kOutOfRange (store dropped)
   bool rowlast[RPW];
@@ -357,8 +384,9 @@ struct TileRunner {
     edge_xy = (xl0 + TILE_X > geo.nx) || (yl0 + TILE_Y > geo.ny);
     xlane = xl0 + lane * VEC;
     yrow0 = yl0 + wave * RPW;
-    // a negative plane stride walks the planes downwards (row streaming)
-    plane_bytes = (int)((geo.plane < 0 ? -geo.plane : geo.plane) * (long long)sizeof(ST));
+    plane_bytes = (int)(geo.plane * (long long)sizeof(ST));
+    lastr = W2D ? geo.rows2d - 1 - (geo.nz - 1) * RPW : RPW - 1;
+    row_bytes = geo.pitch * (int)sizeof(ST);
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int y = yl0 + wave * RPW + r;
@@ -384,11 +412,19 @@ struct TileRunner {
           for (int v = 0; v < NV; ++v) lv[l][s][r][v] = C(0);
   }
 
+  // Bytes of plane q that belong to the field: the whole plane, except the
+  // clipped last row block of a 2D field (rows beyond it read zeros and their
+  // stores are dropped).
+  __device__ __forceinline__ int valid_bytes(int q) const {
+    if constexpr (W2D) return min(RPW, geo.rows2d - q * RPW) * row_bytes;
+    else return plane_bytes;
+  }
+
   // Load plane q into level-0 slot SL; planes outside [zl0, zl1) read zeros.
   template <int SL>
   __device__ __forceinline__ void load_plane(int q) {
     const bool in = q >= zl0 && q < zl1;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(src + (long long)q * geo.plane, in ? plane_bytes : 0);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(src + (long long)q * geo.plane, in ? valid_bytes(q) : 0);
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
 #pragma unroll
@@ -398,7 +434,7 @@ struct TileRunner {
   // Store plane q; planes outside [zo0, zo1) are dropped.
   __device__ __forceinline__ void store_plane(int q, C (&out)[RPW][NV]) {
     const bool in = q >= zo0 && q < zo1;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(dst + (long long)q * geo.plane, in ? plane_bytes : 0);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(dst + (long long)q * geo.plane, in ? valid_bytes(q) : 0);
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
 #pragma unroll
@@ -443,7 +479,12 @@ struct TileRunner {
       }
       // rows above / below this wave's first / last row
       C nrow[NV], srow[NV];
-      if (r == 0) {
+      if (W2D && r == 0) {
+        // the previous plane's last row (its value at a field edge is
+        // replaced by the cell itself below)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) nrow[v] = lv[s - 1][sb][RPW - 1][v];
+      } else if (r == 0) {
         if constexpr (NW > 1) {
           if (wave > 0) {
 #pragma unroll
@@ -460,7 +501,11 @@ struct TileRunner {
 #pragma unroll
         for (int v = 0; v < NV; ++v) nrow[v] = lv[s - 1][sc][r - 1][v];
       }
-      if (r == RPW - 1) {
+      if (W2D && r == RPW - 1) {
+        // the next plane's first row
+#pragma unroll
+        for (int v = 0; v < NV; ++v) srow[v] = lv[s - 1][st][0][v];
+      } else if (r == RPW - 1) {
         if constexpr (NW > 1) {
           if (wave < NW - 1) {
 #pragma unroll
@@ -491,15 +536,25 @@ struct TileRunner {
           const C w = (v > 0) ? cv[gv - 1] : wl;
           C e = (v < VEC - 1) ? cv[gv + 1] : er;
           C so = srow[gv];
-          C b = lv[s - 1][sb][r][gv];
-          C t = lv[s - 1][st][r][gv];
-          if constexpr (EDGE) {
-            if (xlane + g * GSTRIDE + v == geo.nx - 1) e = c;
-            if (rowlast[r]) so = c;
-            if (zb) b = c;
-            if (zt) t = c;
+          if constexpr (W2D) {
+            C no = nrow[gv];
+            if constexpr (EDGE) {
+              if (xlane + g * GSTRIDE + v == geo.nx - 1) e = c;
+              if (zb && r == 0) no = c;         // field row 0
+              if (zt && r == lastr) so = c;     // the field's last row
+            }
+            out[r][gv] = update5<C, FMA>(k, c, w, e, so, no);
+          } else {
+            C b = lv[s - 1][sb][r][gv];
+            C t = lv[s - 1][st][r][gv];
+            if constexpr (EDGE) {
+              if (xlane + g * GSTRIDE + v == geo.nx - 1) e = c;
+              if (rowlast[r]) so = c;
+              if (zb) b = c;
+              if (zt) t = c;
+            }
+            out[r][gv] = update<C, FMA>(k, c, w, e, so, nrow[gv], b, t);
           }
-          out[r][gv] = update<C, FMA>(k, c, w, e, so, nrow[gv], b, t);
         }
       }
     }
@@ -607,6 +662,21 @@ stencil_kernel(const typename Traits<S>::T *__restrict__ src, typename Traits<S>
   const int ty = rest % geo.nty;
   const int tz = rest / geo.nty;
   R tr(src, dst, geo, k, xch, tx, ty, tz);
+  tr.run();
+}
+
+// The 2D 5-point operator on an nx x ny field streamed RPW rows per plane
+// (geo.ny = RPW, geo.nz = planes, geo.rows2d = ny): one wave per workgroup.
+template <int S, int VEC, int T, int G, int RPW, bool FMA, int PF = 1>
+__global__ void __launch_bounds__(64)
+stencil2d_kernel(const typename Traits<S>::T *__restrict__ src, typename Traits<S>::T *__restrict__ dst,
+                 Geometry geo, Coef<typename Traits<S>::C> k) {
+  using R = TileRunner<S, VEC, T, G, RPW, 1, FMA, PF, true>;
+  __shared__ typename Traits<S>::C xch[R::X::kElems];
+  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+  const int tx = lb % geo.ntx;
+  const int tz = lb / geo.ntx;  // one row-block tile per x tile (nty = 1)
+  R tr(src, dst, geo, k, xch, tx, 0, tz);
   tr.run();
 }
 

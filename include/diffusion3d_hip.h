@@ -189,7 +189,8 @@ typedef struct {
   int waves;           /* wavefronts per workgroup                          */
   int tile_x, tile_y;  /* loaded tile extent in cells / rows               */
   int prefetch;        /* planes requested ahead of the one being computed  */
-  int row_stream;      /* 1: streams the rows of nz == 1 fields (2D); such a
+  int row_stream;      /* 1: 2D kernel, streams the rows of nz == 1 fields in
+                          blocks of rows_per_wave rows; such a
                           tile runs only fields with nz == 1 and ct == cb == 0,
                           and is chosen for them when no tile is given      */
 } d3h_tile_desc;

@@ -70,8 +70,8 @@ def test_tile_table():
         assert t["dtype"] in (0, 1, 2, 3) and t["temporal_depth"] in (1, 2, 4, 8)
         assert t["tile_y"] == t["rows_per_wave"] * t["waves"]
         assert t["tile_x"] % 64 == 0
-        if t["row_stream"]:               # streams the rows of nz == 1 fields
-            assert t["tile_y"] == 1 and t["dtype"] == 0
+        if t["row_stream"]:               # 2D: streams blocks of rows of nz == 1 fields
+            assert t["waves"] == 1 and t["tile_y"] == t["rows_per_wave"] and t["dtype"] in (0, 1)
         else:
             assert t["tile_y"] > 2 * t["temporal_depth"] or t["temporal_depth"] == 1
     for dt in (0, 1, 2, 3):

@@ -152,14 +152,14 @@ def test_driver_nd2_matches_family_output(golden2d, tmp_path):
         assert lines[j] == "%.7E" % float(v), j
 
 
-def row_tiles(T=None, arith=d3.ARITH_EXACT):
+def row_tiles(T=None, arith=d3.ARITH_EXACT, dtype=d3.F32):
     return [t for t in d3.tiles() if t["row_stream"] and t["arith"] == arith
-            and (T is None or t["temporal_depth"] == T)]
+            and t["dtype"] == dtype and (T is None or t["temporal_depth"] == T)]
 
 
 @pytest.mark.parametrize("shape", [(300, 77), (1000, 9), (5, 300), (520, 1), (1, 40)])
 def test_every_row_tile_exact_matches_oracle(ctx, shape):
-    """Row-streaming tiles (every depth, chunking) against the oracle's 2D
+    """2D tiles (every depth, chunking) against the oracle's 2D
     operator on noisy fields, counts that are not multiples of the depth."""
     nx, ny = shape
     k = d3.constants2d_f32(nx, ny)
@@ -231,7 +231,7 @@ def test_row_tiles_refuse_3d_operator(ctx):
 
 
 def test_full_size_16384_depths_equal_plain(ctx):
-    """The 2D bench grid (16384^2): every row-streaming depth equals plain
+    """The 2D bench grid (16384^2): every 2D-kernel depth equals plain
     steps bitwise at full size (device-side comparison)."""
     n = 16384
     k = d3.constants2d_f32(n, n)
@@ -248,3 +248,34 @@ def test_full_size_16384_depths_equal_plain(ctx):
                 tm = tb.run(11, c, depth=depth)
                 assert d3.tiles()[tm["tile"]]["row_stream"] == 1
                 assert tb.compare(plain)["max_abs"] == 0.0, depth
+
+
+def test_f64_2d_tiles_equal_the_7_point_tiles(ctx):
+    """FP64 2D tiles (every depth, chunking) on a row-reversed nz == 1 field
+    equal the FP64 7-point tiles with ct = cb = 0 bitwise (positive field, so
+    the dropped 0 * c terms cannot change a sign); the 7-point FP64 kernel is
+    pinned to the oracle elsewhere."""
+    nx, ny = 300, 77
+    k = d3.constants2d_f32(nx, ny)
+    f0 = d3.add_noise_f32(d3.initialize2d_f32(nx, ny, k).reshape(-1), 5, 1e-3).reshape(ny, nx)
+    f0 = f0.astype(np.float64)
+    c = d3.coefficients(k)
+    c[4] = c[5] = 0.0
+    t3 = [t for t in d3.tiles() if t["dtype"] == d3.F64 and t["temporal_depth"] == 1
+          and t["arith"] == d3.ARITH_EXACT and not t["row_stream"]][0]
+    with d3.Field(ctx, nx, ny, 1, d3.F64) as f:
+        f.set_y_reversed(True)
+        f.upload(f0)
+        f.run(11, c, tile=t3["index"])
+        want = f.download()
+    tiles = row_tiles(dtype=d3.F64)
+    assert tiles
+    for t in tiles:
+        for zc in (0, 1, 3):
+            with d3.Field(ctx, nx, ny, 1, d3.F64) as f:
+                f.set_y_reversed(True)
+                f.upload(f0)
+                tm = f.run(11, c, depth=t["temporal_depth"], tile=t["index"], zchunks=zc)
+                got = f.download()
+            assert tm["steps"] == 11
+            assert np.array_equal(got, want), (t, zc)
