@@ -171,7 +171,7 @@ def test_every_row_tile_exact_matches_oracle(ctx, shape):
         T = t["temporal_depth"]
         for count in (5, 11):
             for zc in (0, 1, 3):
-                print("row tile", t["index"], "T", T, shape, count, zc, flush=True)
+                print("2D tile", t["index"], "T", T, shape, count, zc, flush=True)
                 with d3.Field(ctx, nx, ny, 1) as f:
                     f.set_y_reversed(True)
                     f.upload(f0)
@@ -210,7 +210,7 @@ def test_auto_choice_is_row_streaming_and_exact(ctx):
 
 
 def test_row_tiles_refuse_3d_operator(ctx):
-    """A row tile is only for nz == 1 with ct == cb == 0; nz == 1 fields with
+    """A 2D tile is only for nz == 1 with ct == cb == 0; nz == 1 fields with
     t/b coefficients run on the 3D tiles and match the 7-point oracle."""
     t = row_tiles(1)[0]
     with d3.Field(ctx, 64, 16, 4) as f:
