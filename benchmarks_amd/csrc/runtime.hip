@@ -932,9 +932,13 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
   const long long cells = (long long)f->nx * f->ny * f->nz;
   // enough launches for ~2 ms of work at ~1 Tcell/s, at least 4
   const int reps = (int)std::max(4LL, std::min(200LL, 2000000000LL / std::max(cells, 1LL) / T));
+  // a 2D field that 2D kernels can run skips the 3D tiles (one plane per
+  // workgroup: several times slower, and long to time on a large field)
+  const bool only_2d = rows && default_entry(f->dtype, T, fma, true) != nullptr;
   for (int i = 0; i < (int)reg.size(); ++i) {
     const KernelEntry &e = reg[i];
     if (e.dtype != f->dtype || e.T != T || e.fma != fma || (e.rows && !rows)) continue;
+    if (only_2d && !e.rows) continue;
     const int occ = occupancy_of(e);
     if (occ <= 0) continue;
     Geometry g{};
