@@ -9,13 +9,24 @@ before the timed region; the timed region is one synchronous d3h_field_run of
 exactly --steps steps (the library records HIP events on its own stream around
 the launches and synchronises on the last one, so the region is bracketed by
 device synchronisation on both sides).  value = Gcell-updates/s from the host
-wall clock of that call.
+wall clock of that call.  The steps are launched directly (--graph 0): on the
+512^3 grid a hipGraph replay measured 0.2-0.27 ms of fixed cost per call and
+about 1 % more per launch than direct launches, whose fixed cost is ~13 us.
 
 roofline: algorithmic bytes per launch (2 x storage bytes x cells, one read
 and one write per cell, as the reference's GetThroughput) / average launch
-duration from those events, against the 8 TB/s HBM peak.  traffic is read
-from profiles/traffic_<workload>.json (rocprofv3 FETCH_SIZE / WRITE_SIZE),
-when present.
+duration from those events, against the 8 TB/s HBM peak; hbm_gbps_wall /
+frac_wall give the same bytes over the wall-clock time per launch (the
+timing of value).  traffic is read from profiles/traffic_<workload>.json
+(rocprofv3 FETCH_SIZE / WRITE_SIZE), when present.
+
+served (default workload only, --served-requests 0 skips it): the second half
+of the headline metric -- the request server on a bounded sample of the
+served configuration (seeded Poisson stream, dims log-uniform [32, 384],
+50-2000 steps, 70 % FP32 / 30 % BF16 storage with stochastic rounding):
+closed-loop solves/s, p50/p99 latency at --served-rate arrivals/s, solves/s
+of the plain kernel launched per request, whether every result equals the
+per-request result bitwise, and FP32 / BF16 errors against the FP64 GPU run.
 
 error_vs_fp64_gpu_oracle: relative L2 and max-abs difference of the FP32
 result from the same initial field and coefficients advanced in FP64 on the
@@ -237,13 +248,13 @@ def bench_batched(args, d3, ctx, dname):
     }
 
 
-def bench_served(args, d3, ctx):
+def served_measure(d3, ctx, nreq, rates, arith, depth, sweep_steps, rounding):
+    """The request server on nreq seeded requests of the served configuration:
+    closed loop (all submitted at once), open loop at each Poisson rate, the
+    plain kernel per request (alloc, H2D, steps, D2H), bitwise comparison and
+    errors against the FP64 GPU run."""
     from benchmarks_amd import workloads
-    arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
-    depth = args.depth if args.depth is not None else 0
-    nreq = args.requests
     arena = workloads.PinnedArena()
-    rounding = d3.ROUND_STOCHASTIC if args.bf16_rounding == "stochastic" else d3.ROUND_NEAREST
     base = workloads.served_requests(nreq, lam=1.0, rounding=rounding)
     ins, outs = [], []
     for i, p in enumerate(base):
@@ -252,7 +263,7 @@ def bench_served(args, d3, ctx):
         ins.append(a)
         outs.append(arena.empty(a.shape))
     work = sum(p["nx"] * p["ny"] * p["nz"] * p["steps"] for p in base)
-    srv = d3.Server(ctx, arith=arith, depth=depth, sweep_steps=args.sweep_steps)
+    srv = d3.Server(ctx, arith=arith, depth=depth, sweep_steps=sweep_steps)
     # warm-up
     w = srv.submit(base[0], ins[0], d3.F32, outs[0])
     srv.wait(w)
@@ -265,8 +276,8 @@ def bench_served(args, d3, ctx):
     server_outs = [o.copy() for o in outs]
     print("served closed loop: %d requests in %.3f s" % (nreq, t_closed), file=sys.stderr, flush=True)
     # open loop at each arrival rate
-    rates = []
-    for lam in [float(x) for x in args.rates.split(",")]:
+    rates_out = []
+    for lam in rates:
         reqs = workloads.served_requests(nreq, lam=lam, rounding=rounding)
         lat = []
         t_start = time.perf_counter()
@@ -280,10 +291,10 @@ def bench_served(args, d3, ctx):
             lat.append(srv.wait(r))
         t_open = time.perf_counter() - t_start
         lat = np.array(lat)
-        rates.append({"rate_per_s": lam, "solves_per_s": round(nreq / t_open, 2),
-                      "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                      "p99_ms": round(float(np.percentile(lat, 99)), 2)})
-        print("rate %.0f/s: %s" % (lam, rates[-1]), file=sys.stderr, flush=True)
+        rates_out.append({"rate_per_s": lam, "solves_per_s": round(nreq / t_open, 2),
+                          "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 2)})
+        print("rate %.0f/s: %s" % (lam, rates_out[-1]), file=sys.stderr, flush=True)
     st = srv.stats()
     srv.close()
     # baseline: the plain kernel launched per request, one after another
@@ -302,6 +313,39 @@ def bench_served(args, d3, ctx):
     e32 = fp64_errors(d3, ctx, base, ins, server_outs, f32)
     ebf = fp64_errors(d3, ctx, base, ins, server_outs, bf)
     arena.close()
+    return {"requests": nreq, "cell_updates": work, "t_closed": t_closed, "t_base": t_base,
+            "open_loop": rates_out, "results_equal_per_request": bool(same), "server_stats": st,
+            "error_vs_fp64": {"f32": e32, "bf16": ebf}}
+
+
+def served_summary(d3, ctx, args):
+    """Bounded served figures for the default (headline) line."""
+    arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
+    r = served_measure(d3, ctx, args.served_requests, [args.served_rate], arith, 0,
+                       args.sweep_steps, d3.ROUND_STOCHASTIC)
+    n = r["requests"]
+    return {
+        "workload": "%d seeded requests: edge log-uniform [32,384], slab aspect, steps U[50,2000], "
+                    "70%% FP32 / 30%% BF16 storage (stochastic rounding), depth per size class, "
+                    "%d steps per sweep, %s arithmetic" % (n, args.sweep_steps, args.arith),
+        "closed_loop_solves_per_s": round(n / r["t_closed"], 2),
+        "closed_loop_gcell_updates_per_s": round(r["cell_updates"] / r["t_closed"] / 1e9, 2),
+        "per_request_plain_solves_per_s": round(n / r["t_base"], 2),
+        "speedup_vs_per_request": round(r["t_base"] / r["t_closed"], 2),
+        "open_loop": r["open_loop"][0],
+        "results_equal_per_request": r["results_equal_per_request"],
+        "error_vs_fp64": r["error_vs_fp64"],
+    }
+
+
+def bench_served(args, d3, ctx):
+    arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
+    depth = args.depth if args.depth is not None else 0
+    nreq = args.requests
+    rounding = d3.ROUND_STOCHASTIC if args.bf16_rounding == "stochastic" else d3.ROUND_NEAREST
+    r = served_measure(d3, ctx, nreq, [float(x) for x in args.rates.split(",")], arith, depth,
+                       args.sweep_steps, rounding)
+    t_closed, t_base, work = r["t_closed"], r["t_base"], r["cell_updates"]
     return {
         "metric": "served solves/s, p50/p99 latency at equal error",
         "value": round(nreq / t_closed, 2), "unit": "solves/s", "n_gpus": 1, "steps": nreq,
@@ -316,10 +360,10 @@ def bench_served(args, d3, ctx):
         "closed_loop_gcell_updates_per_s": round(work / t_closed / 1e9, 2),
         "per_request_plain_solves_per_s": round(nreq / t_base, 2),
         "speedup_vs_per_request": round(t_base / t_closed, 2),
-        "results_equal_per_request": bool(same),
-        "open_loop": rates,
-        "server_stats": st,
-        "error_vs_fp64": {"f32": e32, "bf16": ebf},
+        "results_equal_per_request": r["results_equal_per_request"],
+        "open_loop": r["open_loop"],
+        "server_stats": r["server_stats"],
+        "error_vs_fp64": r["error_vs_fp64"],
     }
 
 
@@ -333,7 +377,8 @@ def main():
     ap.add_argument("--arith", default="exact", choices=["exact", "fma"])
     ap.add_argument("--depth", type=int, default=None,
                     help="temporal depth (default per workload); 0 = the autotuner's choice")
-    ap.add_argument("--graph", type=int, default=1, help="replay a captured hipGraph (1) or launch")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="replay a captured hipGraph (1) or launch the steps directly (0, default)")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile before timing")
     ap.add_argument("--batch-tune", type=int, default=0,
                     help="batched: run the batch autotuner (depth, tile, z-chunks per size class) "
@@ -346,6 +391,11 @@ def main():
     ap.add_argument("--requests", type=int, default=150, help="served: requests per run")
     ap.add_argument("--rates", default="50,100,150", help="served: Poisson arrival rates (1/s)")
     ap.add_argument("--sweep-steps", type=int, default=32, help="served: steps per scheduling round")
+    ap.add_argument("--served-requests", type=int, default=150,
+                    help="default workload: requests of the bounded served sample (0 = skip)")
+    ap.add_argument("--served-rate", type=float, default=50.0,
+                    help="default workload: Poisson arrival rate (1/s) of the served sample's "
+                         "open-loop run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU (this build has no multi-GPU path); "
@@ -406,17 +456,23 @@ def main():
             return d3.initialize_f64(nx, ny, nz, k)
     coef = d3.coefficients(k)
     field = d3.Field(ctx, nx, ny, nz, dtype)
+    # the initial field stays on the device, so the input is restored right
+    # after the warm-up by a device copy (no host round trip and no idle gap
+    # before the timed region)
+    pristine = d3.Field(ctx, nx, ny, nz, dtype)
     if two_d:
         field.set_y_reversed(True)
-    field.upload(initial())
+        pristine.set_y_reversed(True)
+    pristine.upload(initial())
+    field.copy_from(pristine)
     if args.warmup > 0:
         field.run(args.warmup, coef, depth=depth, arith=arith, graph=False, **rnd)
-    # re-initialise after the warm-up (diffusion/diffusion.h:126-131)
-    field.upload(initial())
     if args.graph:
-        # capture once (untimed), restore the input, then time a replay
+        # capture once (untimed); the input is restored below
+        field.copy_from(pristine)
         field.run(steps, coef, depth=depth, arith=arith, graph=True, **rnd)
-        field.upload(initial())
+    # re-initialise after the warm-up (diffusion/diffusion.h:126-131)
+    field.copy_from(pristine)
     t0 = time.perf_counter()
     tm = field.run(steps, coef, depth=depth, arith=arith, graph=bool(args.graph), **rnd)
     wall = time.perf_counter() - t0
@@ -443,6 +499,7 @@ def main():
             err64 = field.compare(ref)
         print("fp64 oracle: %.1f s" % (time.perf_counter() - t64), file=sys.stderr, flush=True)
     field.close()
+    pristine.close()
 
     bytes_per_launch = 2.0 * esize * cells          # one read + one write per cell
     avg_launch_s = tm["avg_launch_ms"] * 1e-3
@@ -450,6 +507,8 @@ def main():
     # for fused launches the algorithmic bytes are those of the single step
     # sequence they replace (effective bandwidth)
     achieved = bytes_per_launch * steps_per_launch / avg_launch_s / 1e9
+    # the same bytes over the wall-clock time per launch (the timing of value)
+    achieved_wall = bytes_per_launch * steps_per_launch * tm["launches"] / wall / 1e9
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
     # against the copy probe) when the profile is of the kernel that ran here
     traffic = None
@@ -471,6 +530,11 @@ def main():
         grid = tm["blocks"] * 64 * t["waves"]
         if prof.get("kernel") == name and prof.get("grid_size", grid) == grid:
             traffic = prof.get("hbm_bytes_per_launch")
+
+    served = None
+    if args.workload == "plain512_f32" and args.served_requests > 0:
+        served = served_summary(d3, ctx, args)
+        print("served:", served, file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -515,11 +579,14 @@ def main():
         "cpu_baseline": cpu,
         "kernel_ms": round(tm["kernel_ms"], 3),
         "avg_launch_ms": round(tm["avg_launch_ms"], 5),
+        "hbm_gbps_wall": round(achieved_wall, 1),
+        "frac_wall": round(achieved_wall / HBM_PEAK_GBPS, 4),
         "copy_probe_gbps": round(probe["gbps"], 1),
         "frac_of_copy_probe": round(achieved / probe["gbps"], 4),
         "error_vs_analytical_fp64": {"rms": err["rms"], "max_abs": err["max_abs"]},
         "error_vs_fp64_gpu_oracle": ({"rel_l2": err64["rel_l2"], "max_abs": err64["max_abs"]}
                                      if err64 else None),
+        "served": served,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

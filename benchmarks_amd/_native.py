@@ -84,6 +84,7 @@ SIGNATURES = {
     "d3h_field_dims": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int)]),
     "d3h_field_upload": (c_int, [c_void_p, c_void_p, c_int]),
     "d3h_field_download": (c_int, [c_void_p, c_void_p, c_int]),
+    "d3h_field_copy": (c_int, [c_void_p, c_void_p]),
     "d3h_field_save": (c_int, [c_void_p, c_char_p]),
     "d3h_field_load": (c_int, [c_void_p, c_char_p]),
     "d3h_raw_write": (c_int, [c_char_p, c_void_p, c_int, c_int, c_int, c_int, c_uint64, c_int]),

@@ -44,7 +44,7 @@ const KernelEntry *pick_batch_entry(d3h_context *ctx, int nx, int ny, int nz, in
       auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz, batch));
       if (it == ctx->tuned.end()) continue;
       const TuneVal &v = it->second;
-      const KernelEntry *e = find_entry(dtype, T, fma, v.vec, v.G, v.RPW, v.NW, v.pf);
+      const KernelEntry *e = find_entry(dtype, T, fma, v.vec, v.G, v.RPW, v.NW, v.pf, 0);
       if (e && e->launch_batch) {
         if (zchunks && batch) *zchunks = v.zchunks;
         return e;
@@ -469,7 +469,7 @@ int d3h_batch_autotune(d3h_batch *b, int arith, int *classes, double *ms_before,
   std::vector<char> host;
   auto set_entry = [&](TuneKey k, const KernelEntry &e, int zc, double t) {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    ctx->tuned[k] = TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, zc, t, e.T};
+    ctx->tuned[k] = TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, zc, t, e.T, 0};
   };
   auto with_depth = [](TuneKey k, int T) {
     k.T = T;

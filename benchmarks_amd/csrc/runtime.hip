@@ -129,12 +129,17 @@ TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz, int batch) {
   return TuneKey{dtype, T, fma, ilog2ceil(nx), ilog2ceil(ny), ilog2ceil(nz), batch};
 }
 
-const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf) {
+const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf,
+                              int rows) {
   for (const auto &e : registry())
     if (e.dtype == dtype && e.T == T && e.fma == fma && e.vec == vec && e.G == G && e.RPW == RPW &&
-        e.NW == NW && e.pf == pf)
+        e.NW == NW && e.pf == pf && (e.rows != 0) == (rows != 0))
       return &e;
   return nullptr;
+}
+
+const KernelEntry *find_entry(int dtype, int fma, const TuneVal &v) {
+  return find_entry(dtype, v.T, fma, v.vec, v.G, v.RPW, v.NW, v.pf, v.rows);
 }
 
 bool rows_ok(const d3h_field *f, const double coef[7]) {
@@ -209,7 +214,7 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
       auto it = ctx->tuned.find(tune_key(dtype, 0, fma, nx, ny, nz));
       if (it != ctx->tuned.end()) {
         const TuneVal &v = it->second;
-        const KernelEntry *e = find_entry(dtype, v.T, fma, v.vec, v.G, v.RPW, v.NW, v.pf);
+        const KernelEntry *e = find_entry(dtype, fma, v);
         if (e && (!e->rows || rows)) {
           *entry = e;
           *zchunks = cfg.zchunks > 0 ? cfg.zchunks : v.zchunks;
@@ -220,9 +225,8 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
     }
     auto it = ctx->tuned.find(tune_key(dtype, T, fma, nx, ny, nz));
     if (it != ctx->tuned.end()) {
-      const KernelEntry *e = find_entry(dtype, T, fma, it->second.vec, it->second.G, it->second.RPW,
-                                        it->second.NW, it->second.pf);
-      if (e && (!e->rows || rows)) {
+      const KernelEntry *e = find_entry(dtype, fma, it->second);
+      if (e && e->T == T && (!e->rows || rows)) {
         *entry = e;
         *zchunks = cfg.zchunks > 0 ? cfg.zchunks : it->second.zchunks;
         return D3H_OK;
@@ -659,6 +663,23 @@ int d3h_field_download(d3h_field *f, void *host, int host_dtype) {
   return D3H_OK;
 }
 
+int d3h_field_copy(d3h_field *dst, const d3h_field *src) {
+  if (!dst || !src) return fail(D3H_ERR_INVALID, "field is NULL");
+  if (dst == src) return D3H_OK;
+  if (dst->ctx != src->ctx) return fail(D3H_ERR_INVALID, "fields of different contexts");
+  if (dst->nx != src->nx || dst->ny != src->ny || dst->nz != src->nz || dst->dtype != src->dtype)
+    return fail(D3H_ERR_INVALID, "copy between a %dx%dx%d field of dtype %d and a %dx%dx%d field of "
+                "dtype %d", src->nx, src->ny, src->nz, src->dtype, dst->nx, dst->ny, dst->nz, dst->dtype);
+  if (dst->yrev != src->yrev) return fail(D3H_ERR_INVALID, "fields store their rows in different orders");
+  d3h_context *ctx = dst->ctx;
+  D3H_HIP(hipSetDevice(ctx->device));
+  D3H_HIP(hipMemcpyAsync(dst->buf[dst->cur], src->buf[src->cur], src->bytes, hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  D3H_HIP(hipStreamSynchronize(ctx->stream));
+  dst->steps_done = src->steps_done;
+  return D3H_OK;
+}
+
 // ---- raw field files ------------------------------------------------------------
 // 40-byte little-endian header ("D3HFIELD", version 1, dtype, nx, ny, nz,
 // flags, steps since upload), then the dense field in host row order.
@@ -930,11 +951,22 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
   const auto &reg = registry();
   const bool rows = rows_ok(f, coef);
   const long long cells = (long long)f->nx * f->ny * f->nz;
-  // enough launches for ~2 ms of work at ~1 Tcell/s, at least 4
-  const int reps = (int)std::max(4LL, std::min(200LL, 2000000000LL / std::max(cells, 1LL) / T));
+  // enough launches for ~4 ms of work at ~1 Tcell/s, at least 4; every
+  // candidate is timed in kPasses interleaved passes and keeps its fastest,
+  // so one noisy measurement cannot pick a tile that is slower in steady
+  // state (tiles of one depth are often within 1-2 % of each other)
+  constexpr int kPasses = 2;
+  const int reps = (int)std::max(4LL, std::min(200LL, 4000000000LL / std::max(cells, 1LL) / T));
   // a 2D field that 2D kernels can run skips the 3D tiles (one plane per
   // workgroup: several times slower, and long to time on a large field)
   const bool only_2d = rows && default_entry(f->dtype, T, fma, true) != nullptr;
+  struct Cand {
+    int entry;
+    Geometry g;
+    unsigned blocks;
+    double best;
+  };
+  std::vector<Cand> cands;
   for (int i = 0; i < (int)reg.size(); ++i) {
     const KernelEntry &e = reg[i];
     if (e.dtype != f->dtype || e.T != T || e.fma != fma || (e.rows && !rows)) continue;
@@ -947,32 +979,47 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
     // planes (rows, row blocks) streamed
     const int nstream = e.rows ? (f->ny + e.RPW - 1) / e.RPW : f->nz;
     // few long streams (1-4 chunks) up to several waves of workgroups
-    std::vector<int> cands = {1, 2, 3, 4, g.nzc, g.nzc * 2, g.nzc * 4, std::max(1, g.nzc / 2)};
-    std::sort(cands.begin(), cands.end());
-    cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
-    for (int zc : cands) {
+    std::vector<int> zcs = {1, 2, 3, 4, g.nzc, g.nzc * 2, g.nzc * 4, std::max(1, g.nzc / 2)};
+    std::sort(zcs.begin(), zcs.end());
+    zcs.erase(std::unique(zcs.begin(), zcs.end()), zcs.end());
+    int last_nzc = -1;
+    for (int zc : zcs) {
       if (zc > nstream) continue;
       Geometry g2{};
       unsigned nb2 = 0;
       if (field_geometry(e, f, zc, &g2, &nb2) != D3H_OK) continue;
-      launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);  // warm-up
+      if (g2.nzc == last_nzc) continue;  // same launch as the previous count
+      last_nzc = g2.nzc;
+      cands.push_back(Cand{i, g2, nb2, 1e300});
+    }
+  }
+  // 16-bit storage is timed with stochastic rounding, the mode this problem
+  // needs (nearest even stalls its small per-step changes) and the costlier one
+  if (f->dtype == D3H_BF16 || f->dtype == D3H_F16)
+    for (Cand &c : cands) c.g.sr = 1;
+  for (int pass = 0; pass < kPasses; ++pass) {
+    for (Cand &c : cands) {
+      const KernelEntry &e = reg[c.entry];
+      launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], c.g, coef, c.blocks, ctx->stream);  // warm-up
       f->cur ^= 1;
       (void)hipEventRecord(ctx->ev0, ctx->stream);
       for (int r = 0; r < reps; ++r) {
-        launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], g2, coef, nb2, ctx->stream);
+        launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], c.g, coef, c.blocks, ctx->stream);
         f->cur ^= 1;
       }
       (void)hipEventRecord(ctx->ev1, ctx->stream);
       if (hipEventSynchronize(ctx->ev1) != hipSuccess || hipGetLastError() != hipSuccess)
-        return fail(D3H_ERR_HIP, "autotune launch of tile %d failed", i);
+        return fail(D3H_ERR_HIP, "autotune launch of tile %d failed", c.entry);
       float ms = 0;
       (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-      const double per_step = ms / reps / T;
-      if (per_step < *best) {
-        *best = per_step;
-        *bi = i;
-        *bz = g2.nzc;
-      }
+      c.best = std::min(c.best, (double)ms / reps / T);
+    }
+  }
+  for (const Cand &c : cands) {
+    if (c.best < *best) {
+      *best = c.best;
+      *bi = c.entry;
+      *bz = c.g.nzc;
     }
   }
   return D3H_OK;
@@ -1007,7 +1054,8 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
     const KernelEntry &e = reg[bi];
     {
       std::lock_guard<std::mutex> lk(ctx->mu);
-      ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] = TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz, best, T};
+      ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] =
+          TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz, best, T, e.rows};
     }
     if (best < best_all) {
       best_all = best;
@@ -1022,7 +1070,7 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
     const KernelEntry &e = reg[bi_all];
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->tuned[tune_key(dtype, 0, fma, nx, ny, nz)] =
-        TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz_all, best_all, e.T};
+        TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz_all, best_all, e.T, e.rows};
   }
   if (best_tile) *best_tile = bi_all;
   if (best_zchunks) *best_zchunks = bz_all;
@@ -1035,15 +1083,15 @@ int d3h_autotune_save(d3h_context *ctx, const char *path) {
   std::ofstream os(path);
   if (!os) return fail(D3H_ERR_INVALID, "cannot write %s", path);
   os << "# dtype depth fma log2nx log2ny log2nz vec groups rows_per_wave waves prefetch zchunks "
-        "ms_per_step entry_depth batch   (depth 0 = the depth chosen for the size class; "
-        "batch 1 = chosen for batched sweeps)\n";
+        "ms_per_step entry_depth batch rows   (depth 0 = the depth chosen for the size class; "
+        "batch 1 = chosen for batched sweeps; rows 1 = a 2D entry)\n";
   std::lock_guard<std::mutex> lk(ctx->mu);
   for (const auto &kv : ctx->tuned) {
     const auto &k = kv.first;
     const auto &v = kv.second;
     os << k.dtype << ' ' << k.T << ' ' << k.fma << ' ' << k.cx << ' ' << k.cy << ' ' << k.cz << ' '
        << v.vec << ' ' << v.G << ' ' << v.RPW << ' ' << v.NW << ' ' << v.pf << ' ' << v.zchunks << ' '
-       << v.ms_per_step << ' ' << v.T << ' ' << k.batch << '\n';
+       << v.ms_per_step << ' ' << v.T << ' ' << k.batch << ' ' << v.rows << '\n';
   }
   return os ? D3H_OK : fail(D3H_ERR_INVALID, "write to %s failed", path);
 }
@@ -1063,15 +1111,19 @@ int d3h_autotune_load(d3h_context *ctx, const char *path) {
           v.pf >> v.zchunks >> v.ms_per_step))
       return fail(D3H_ERR_INVALID, "malformed line in %s: %s", path, line.c_str());
     // entry depth (absent in tables without depth-0 lines: the key's depth),
-    // then the batch flag (absent in tables without batched choices: 0)
+    // the batch flag (absent in tables without batched choices: 0), then the
+    // 2D flag (absent in tables written before 2D entries were told apart: 0)
     if (!(ss >> v.T)) v.T = k.T;
     else if (!(ss >> k.batch)) k.batch = 0;
+    else if (!(ss >> v.rows)) v.rows = 0;
     if (k.batch != 0 && k.batch != 1)
       return fail(D3H_ERR_INVALID, "bad batch flag in %s: %s", path, line.c_str());
+    if (v.rows != 0 && v.rows != 1)
+      return fail(D3H_ERR_INVALID, "bad 2D flag in %s: %s", path, line.c_str());
     if (v.T <= 0 || (k.T != 0 && v.T != k.T))
       return fail(D3H_ERR_INVALID, "bad entry depth in %s: %s", path, line.c_str());
     // entries no longer compiled are skipped
-    if (find_entry(k.dtype, v.T, k.fma, v.vec, v.G, v.RPW, v.NW, v.pf)) ctx->tuned[k] = v;
+    if (find_entry(k.dtype, k.fma, v)) ctx->tuned[k] = v;
   }
   return D3H_OK;
 }

@@ -60,12 +60,18 @@ struct TuneVal {
   int zchunks;
   double ms_per_step;
   int T;                    // depth of the entry; under a depth-0 key, the tuned depth
+  int rows;                 // 1: a 2D entry (stencil2d_kernel); 3D tiles and 2D
+                            // entries can share every other parameter
 };
 // Size class of a grid: ceil(log2) of each dimension.  Depth 0 keys hold the
 // depth the autotuner chose for the class (d3h_autotune with depth 0).
 TuneKey tune_key(int dtype, int T, int fma, int nx, int ny, int nz, int batch = 0);
-// The compiled entry with these parameters, or nullptr.
-const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf);
+// The compiled entry with these parameters, or nullptr.  rows selects 2D
+// entries or 3D tiles (a 3D tile can have exactly a 2D entry's parameters).
+const KernelEntry *find_entry(int dtype, int T, int fma, int vec, int G, int RPW, int NW, int pf,
+                              int rows);
+// The entry a tune-table value names (its depth, vec, G, RPW, NW, pf, rows).
+const KernelEntry *find_entry(int dtype, int fma, const TuneVal &v);
 // The tuned depth of a size class, or `dflt` when the class has none.
 int tuned_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, int dflt);
 

@@ -183,6 +183,11 @@ struct Pack {
   __device__ static __forceinline__ void store(__amdgpu_buffer_rsrc_t rs, int off, const C *v) {
     uint32_t w[kWords];
     pack(v, w);
+    store_words(rs, off, w);
+  }
+  // Store already packed words (kWords of them).
+  __device__ static __forceinline__ void store_words(__amdgpu_buffer_rsrc_t rs, int off,
+                                                     const uint32_t *w) {
     if constexpr (kBytes == 16) {
       u32x4 u = {w[0], w[1], w[2], w[3]};
       __builtin_amdgcn_raw_buffer_store_b128(u, rs, off, 0, kStoreAux);
@@ -193,16 +198,25 @@ struct Pack {
   }
 };
 
-// Deterministic stochastic rounding FP32 -> BF16.  16 random bits are added
-// below the BF16 mantissa before truncation, so x rounds away from zero with
-// probability equal to its distance from the lower-magnitude neighbour (in
-// units of that ulp): unbiased, and reproducible on any device or on the
-// host.  The bits come from a 32-bit hash of (seed, cell pair, step): cells
-// x = 2i and 2i+1 of a row share pair i (pair index (z*ny + y) * ceil(nx/2) + i)
-// and take the low and high halves.  The hash is the "lowbias32" integer mixer
-// (C. Wellons, hash-prospector) of x = pair * 0x9E3779B1 + step * 0x85EBCA77
-// + seed; sr_mix gets that sum.
-__device__ __forceinline__ uint32_t sr_mix(uint32_t x) {
+// Deterministic stochastic rounding of 16-bit storage.  Cell (x, y, z) at
+// step t takes 16 random bits r from
+//   base = lowbias32(row * 0x9E3779B1 + t * 0x85EBCA77 + seed),  row = z*ny + y
+//   h    = base + (x / 2) * 0x9E3779B9
+// (lowbias32: C. Wellons' hash-prospector integer mixer): the high half of h
+// for even x, the low half for odd x.  base is uniform per wave (rows belong
+// to waves), so it is computed on the scalar unit; a cell pair costs one add.
+// gfx950 rounds with v_cvt_sr_bf16_f32 / v_cvt_sr_f16_f32, which take the
+// high 16 bits of their random operand: h for the even cell, h << 16 for the
+// odd one.  BF16: r is added below the mantissa and the sum truncated (inf and
+// nan round to nearest).  FP16: |x| + (r >> 3) FP16 ulps / 2^13 truncated to
+// the FP16 grid, subnormals included.  Unbiased, and reproducible bitwise on
+// the host (the oracle restates these rules; a probe over special and random
+// operands pinned them on an MI355X).
+constexpr uint32_t kSrRowMul = 0x9E3779B1u;
+constexpr uint32_t kSrStepMul = 0x85EBCA77u;
+constexpr uint32_t kSrPairWeyl = 0x9E3779B9u;
+
+__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
   x ^= x >> 15;
@@ -210,33 +224,24 @@ __device__ __forceinline__ uint32_t sr_mix(uint32_t x) {
   x ^= x >> 16;
   return x;
 }
-// r: 16 random bits.  Infinities pass unchanged through the add-and-truncate
-// (zero mantissa, r < 2^16); NaNs are quieted like to_bf16 does.
-__device__ __forceinline__ float sr_bf16(float x, uint32_t r) {
-  const uint32_t u = __float_as_uint(x);
-  const uint32_t t = (u + r) & 0xffff0000u;
-  const uint32_t q = (u & 0xffff0000u) | 0x00400000u;  // to_bf16 of a NaN, in place
-  return __uint_as_float((u & 0x7fffffffu) > 0x7f800000u ? q : t);
-}
 
-// Stochastic rounding FP32 -> FP16 with the same 16 random bits r: inside the
-// FP16 normal range (2^-14 <= |x| < 2^16) the low 13 bits of r are added below
-// the FP16 mantissa before truncation (the result is exactly representable,
-// or 2^16 which converts to inf); outside it (zero, subnormal, inf, nan) the
-// value rounds to nearest even.
-__device__ __forceinline__ float sr_f16(float x, uint32_t r) {
-  const uint32_t u = __float_as_uint(x);
-  const uint32_t e = (u >> 23) & 0xffu;
-  const float t = __uint_as_float((u + (r & 0x1fffu)) & 0xffffe000u);
-  return f16_bits_to_f32(to_f16((e >= 113u && e <= 142u) ? t : x));
-}
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
-// Round a compute value to storage precision and back (identity for f32/f64).
+// Two FP32 values (even and odd cell of one pair) stochastically rounded to
+// packed 16-bit storage with the pair's random word h.
 template <int S>
-__device__ __forceinline__ typename Traits<S>::C quantize(typename Traits<S>::C x) {
-  if constexpr (S == kBF16) return __uint_as_float(cvt_pk_bf16(x, 0.0f) << 16);
-  else if constexpr (S == kF16) return f16_bits_to_f32(to_f16(x));
-  else return x;
+__device__ __forceinline__ uint32_t sr_pack_pair(float lo, float hi, uint32_t h) {
+  if constexpr (S == kBF16) {
+    bf16x2 v = {};
+    v = __builtin_amdgcn_cvt_sr_bf16_f32(v, lo, h, false);
+    v = __builtin_amdgcn_cvt_sr_bf16_f32(v, hi, h << 16, true);
+    return __builtin_bit_cast(uint32_t, v);
+  } else {
+    f16x2 v = {};
+    v = __builtin_amdgcn_cvt_sr_f16_f32(v, lo, h, false);
+    v = __builtin_amdgcn_cvt_sr_f16_f32(v, hi, h << 16, true);
+    return __builtin_bit_cast(uint32_t, v);
+  }
 }
 
 // ---- cross-lane moves -------------------------------------------------------
@@ -364,6 +369,7 @@ struct TileRunner {
   int stoff[RPW][G];   // byte offset in a plane, or kOutOfRange (store dropped)
   bool rowlast[RPW];
   bool rowhalo[RPW];    // row is not stored (uniform per wave)
+  uint32_t sr_lane[G];  // 16-bit storage: Weyl offset of the lane's first x pair per group
 
   __device__ __forceinline__ TileRunner(const ST *s_, ST *d_, const Geometry &g, const Coef<C> &k_,
                                         C *x_, int tx, int ty, int tz)
@@ -402,6 +408,11 @@ struct TileRunner {
         stoff[r][g] = (rowst && xv >= xo0 && xv < xo1s) ? off : kOutOfRange;
       }
     }
+    if constexpr (S == kBF16 || S == kF16) {
+      // xlane is a multiple of VEC (even), so a lane's x pairs are its own
+#pragma unroll
+      for (int g = 0; g < G; ++g) sr_lane[g] = ((uint32_t)(xlane + g * GSTRIDE) >> 1) * kSrPairWeyl;
+    }
 #pragma unroll
     for (int l = 0; l < T; ++l)
 #pragma unroll
@@ -439,6 +450,17 @@ struct TileRunner {
     for (int r = 0; r < RPW; ++r)
 #pragma unroll
       for (int g = 0; g < G; ++g) P::store(rs, stoff[r][g], &out[r][g * VEC]);
+  }
+
+  // Store plane q from packed 16-bit words (NV / 2 per row); planes outside
+  // [zo0, zo1) are dropped.
+  __device__ __forceinline__ void store_plane_words(int q, const uint32_t (&w)[RPW][NV / 2]) {
+    const bool in = q >= zo0 && q < zo1;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(dst + (long long)q * geo.plane, in ? valid_bytes(q) : 0);
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+      for (int g = 0; g < G; ++g) P::store_words(rs, stoff[r][g], &w[r][g * VEC / 2]);
   }
 
   // First / last valid plane of level s in this chunk.
@@ -559,40 +581,55 @@ struct TileRunner {
       }
     }
     if constexpr (S == kBF16 || S == kF16) {
+      // 16-bit storage: round every row to packed storage words (nearest even
+      // or stochastic; the branch on geo.sr only touches registers), then
+      // either unpack them as the next level or store them.  One store site,
+      // so the memory instructions of an iteration do not depend on the mode.
+      uint32_t wv[RPW][NV / 2];
       if (geo.sr) {
-        const uint32_t step = (uint32_t)(geo.step0 + s - 1);
+        const uint32_t sbase = (uint32_t)(geo.step0 + s - 1) * kSrStepMul + geo.seed;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-          if (s == T && kSkipHaloRows && rowhalo[r]) continue;
-          const uint32_t prow =
-              ((uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r)) * ((uint32_t)(geo.nx + 1) >> 1);
-          const uint32_t sbase = step * 0x85EBCA77u + geo.seed;
+          if (s == T && kSkipHaloRows && rowhalo[r]) {
+#pragma unroll
+            for (int w = 0; w < NV / 2; ++w) wv[r][w] = 0u;  // the store is dropped
+            continue;
+          }
+          // uniform per wave: the scalar unit computes it
+          const uint32_t row = (uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r);
+          const uint32_t base = lowbias32(row * kSrRowMul + sbase);
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            // xlane is a multiple of VEC (even), so a lane's pairs are its own
-            const uint32_t x0 = (prow + (uint32_t)(xlane + g * GSTRIDE) / 2u) * 0x9E3779B1u + sbase;
+            const uint32_t hb = base + sr_lane[g];
 #pragma unroll
-            for (int v = 0; v < VEC; v += 2) {
-              const uint32_t h = sr_mix(x0 + (uint32_t)(v / 2) * 0x9E3779B1u);
-              if constexpr (S == kBF16) {
-                out[r][g * VEC + v] = sr_bf16(out[r][g * VEC + v], h & 0xffffu);
-                out[r][g * VEC + v + 1] = sr_bf16(out[r][g * VEC + v + 1], h >> 16);
-              } else {
-                out[r][g * VEC + v] = sr_f16(out[r][g * VEC + v], h & 0xffffu);
-                out[r][g * VEC + v + 1] = sr_f16(out[r][g * VEC + v + 1], h >> 16);
-              }
-            }
+            for (int v = 0; v < VEC; v += 2)
+              wv[r][(g * VEC + v) / 2] = sr_pack_pair<S>(out[r][g * VEC + v], out[r][g * VEC + v + 1],
+                                                          hb + (uint32_t)(v / 2) * kSrPairWeyl);
           }
         }
+      } else {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+          for (int g = 0; g < G; ++g) P::pack(&out[r][g * VEC], &wv[r][g * VEC / 2]);
       }
-    }
-    if constexpr (s < T) {
+      if constexpr (s < T) {
 #pragma unroll
-      for (int r = 0; r < RPW; ++r)
+        for (int r = 0; r < RPW; ++r)
 #pragma unroll
-        for (int v = 0; v < NV; ++v) lv[s][sc][r][v] = quantize<S>(out[r][v]);
+          for (int g = 0; g < G; ++g) P::unpack(&wv[r][g * VEC / 2], &lv[s][sc][r][g * VEC]);
+      } else {
+        store_plane_words(p, wv);
+      }
     } else {
-      store_plane(p, out);
+      if constexpr (s < T) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+          for (int v = 0; v < NV; ++v) lv[s][sc][r][v] = out[r][v];
+      } else {
+        store_plane(p, out);
+      }
     }
   }
 

@@ -214,6 +214,11 @@ class Field:
                                                                          DTYPE_NAMES[host_dtype]))
         check(lib().d3h_field_upload(self.handle, _dptr(host), host_dtype), "d3h_field_upload")
 
+    def copy_from(self, other):
+        """Device-to-device copy of `other`'s current contents (same dims and
+        dtype) into this field, with its step count."""
+        check(lib().d3h_field_copy(self.handle, other.handle), "d3h_field_copy")
+
     def save(self, path):
         """Bit-exact raw field file (header + dense data in the field's dtype)."""
         check(lib().d3h_field_save(self.handle, str(path).encode()), "d3h_field_save")
@@ -267,7 +272,8 @@ class Field:
 def run_kernel2d(count, f1, coef):
     """The diffusion/ family's 2D RunKernel(count) (diffusion/diffusion_openmp.cc:59-82)
     on a float32 (ny, nx) array, in place: an nx x ny x 1 device field with
-    reversed rows, advanced by the 7-point kernel with ct = cb = 0."""
+    reversed rows and ct = cb = 0, which the library advances with its 2D
+    5-point kernel (stencil2d_kernel)."""
     ny, nx = f1.shape
     c = [float(v) for v in coef]
     c[4] = c[5] = 0.0

@@ -35,23 +35,31 @@ enum {
 };
 
 /* Storage types.  FP32 and FP64 compute in their own precision; BF16 and
- * FP16 are storage formats with FP32 arithmetic and round-to-nearest-even. */
+ * FP16 are storage formats with FP32 arithmetic, rounded to storage after
+ * every step (D3H_ROUND_*). */
 enum { D3H_F32 = 0, D3H_F64 = 1, D3H_BF16 = 2, D3H_F16 = 3 };
 
 /* Arithmetic modes.  EXACT: the reference's 7-term expression with every
  * multiply and add rounded separately, in the reference's order -- bitwise
- * equal to the reference CPU build.  FMA: the same expression contracted the
- * way a contracting compiler does it, fma(cc,c,cw*w) then a chain of fmas. */
+ * equal to the reference CPU build; the parity mode.  FMA: the same
+ * expression contracted the way a contracting compiler does it,
+ * fma(cc,c,cw*w) then a chain of fmas.  FMA is NOT a conforming mode: its
+ * rounding drifts systematically from the reference CPU path (1.04e-6
+ * relative L2 after 100 steps at 128^3, above the 1e-6 parity bound), though
+ * it stays closer to the exact PDE solution. */
 enum { D3H_ARITH_EXACT = 0, D3H_ARITH_FMA = 1 };
 
 /* Rounding of 16-bit storage.  NEAREST: round to nearest even after every
  * step.  STOCHASTIC: round up or down with probability given by the distance
- * to the two neighbours, using a deterministic hash of (seed, cell, step
- * number) -- unbiased, so the small per-step changes of this problem are not
- * lost to rounding, and reproducible bitwise.  For FP16 it applies in the FP16
- * normal range; zeros, subnormals, inf and nan round to nearest.  Step numbers count from the
- * field's last upload, from 0 in every d3h_batch_run, and from a server
- * request's first step. */
+ * to the two neighbours -- unbiased, so the small per-step changes of this
+ * problem are not lost to rounding, and reproducible bitwise.  Cell (x, y, z)
+ * at step t uses 16 bits of h = lowbias32(row * 0x9E3779B1 + t * 0x85EBCA77 +
+ * seed) + (x / 2) * 0x9E3779B9 with row = z*ny + y (the high half for even x,
+ * the low half for odd x), applied by gfx950's v_cvt_sr_bf16_f32 /
+ * v_cvt_sr_f16_f32 (FP16: subnormals round stochastically too; inf and nan
+ * round to nearest).  Step numbers count from the field's last upload (or
+ * the source's count after d3h_field_copy / d3h_field_load), from 0 in every
+ * d3h_batch_run, and from a server request's first step. */
 enum { D3H_ROUND_NEAREST = 0, D3H_ROUND_STOCHASTIC = 1 };
 
 /* ---- drop-in variant ABI ---------------------------------------------------
@@ -142,6 +150,12 @@ int d3h_field_dims(const d3h_field *f, int *nx, int *ny, int *nz, int *dtype);
  * (converted on the device, round-to-nearest-even when narrowing). */
 int d3h_field_upload(d3h_field *f, const void *host, int host_dtype);
 int d3h_field_download(d3h_field *f, void *host, int host_dtype);
+/* Device-to-device copy of src's current contents into dst (same dims, dtype
+ * and row order; different fields of one context); dst also takes src's step
+ * count, so stochastic rounding continues src's step numbers.  Synchronous.
+ * Restores a field without a host round trip (no counterpart in the
+ * reference, whose variants re-initialise on the host). */
+int d3h_field_copy(d3h_field *dst, const d3h_field *src);
 /* Raw field files: a 40-byte header ("D3HFIELD", version, dtype, nx, ny, nz,
  * flags, steps since upload) and the dense field in the field's dtype and the
  * host's row order -- bit-exact, unlike the reference's "%.7E" text dump
@@ -202,7 +216,9 @@ int d3h_tile_info(int index, d3h_tile_desc *out);
  * saved to / loaded from a text file.  best_tile / best_zchunks may be NULL.
  * temporal_depth 0 searches depths 1, 2, 4 and 8 and also remembers the
  * fastest depth for the class, used by runs, batches and servers given depth
- * 0; best_tile then names the chosen depth's tile (d3h_tile_info). */
+ * 0; best_tile then names the chosen depth's tile (d3h_tile_info).  Every
+ * candidate is timed twice, interleaved with the others, and keeps its
+ * faster time; BF16/FP16 storage is timed with stochastic rounding. */
 int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype,
                  int temporal_depth, int arith, int *best_tile, int *best_zchunks,
                  double *best_ms_per_step);
@@ -274,7 +290,10 @@ int d3h_batch_download_all(d3h_batch *b, void *const *hosts, int host_dtype);
 /* Run every problem to its step count.  arith as above; temporal_depth 1, 2,
  * 4 or 8, or 0 = the depth d3h_autotune(..., 0, ...) chose for each problem's
  * size class, else 2 (FP32/FP64 storage) or 1 (BF16/FP16 storage);
- * use_graph 1 = capture + replay. */
+ * use_graph 1 = capture + replay.  Stochastic rounding numbers the steps of
+ * every call from 0: two consecutive runs of N steps reuse the same random
+ * bits per step and differ from one field run of 2N steps (which continues
+ * its step numbers); upload a problem again before running it again. */
 int d3h_batch_run(d3h_batch *b, int arith, int temporal_depth, int use_graph,
                   d3h_timing *timing);
 /* Batch autotuner (the reference's tile sweep, diffusion3d/build.py:6-44,

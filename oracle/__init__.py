@@ -45,7 +45,8 @@ def lib():
             "oracle_f32_to_lp": [_f, _u16, ctypes.c_long, ctypes.c_int],
             "oracle_run_bf16_sr": [_u16, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int,
                                    ctypes.c_uint32, ctypes.c_int],
-            "oracle_sr_hash": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32],
+            "oracle_sr_hash": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32],
+            "oracle_sr_round": [ctypes.c_float, ctypes.c_uint32, ctypes.c_int],
             "oracle_run_lp_sr": [_u16, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_uint32, ctypes.c_int],
             "oracle_lp_to_f32": [_u16, _f, ctypes.c_long, ctypes.c_int],
@@ -62,7 +63,7 @@ def lib():
             "oracle_diff_norms_f32": [_f, _f, ctypes.c_long, _d],
             "oracle_diff_norms_f64": [_d, _d, ctypes.c_long, _d],
         }
-        res = {"oracle_sr_hash": ctypes.c_uint32, "oracle_accuracy_f32": ctypes.c_float, "oracle_accuracy2d_f32": ctypes.c_float, "oracle_rms_vs_analytical_f32": ctypes.c_double,
+        res = {"oracle_sr_hash": ctypes.c_uint32, "oracle_sr_round": ctypes.c_uint16, "oracle_accuracy_f32": ctypes.c_float, "oracle_accuracy2d_f32": ctypes.c_float, "oracle_rms_vs_analytical_f32": ctypes.c_double,
                "oracle_rms_vs_analytical_f64": ctypes.c_double}
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -173,8 +174,15 @@ def run_lp_sr(bits, coef, count, kind, seed, step0=0):
     return a
 
 
-def sr_hash(seed, cell, step):
-    return int(lib().oracle_sr_hash(seed, cell, step))
+def sr_hash(seed, row, pair, step):
+    """The 32 random bits of x pair `pair` of row `row` (= z*ny + y) at `step`."""
+    return int(lib().oracle_sr_hash(seed, row, pair, step))
+
+
+def sr_round(value, r, kind):
+    """One float32 value stochastically rounded to 16 bits with random bits r
+    (kind 0 BF16, 1 FP16): the bit pattern."""
+    return int(lib().oracle_sr_round(float(value), r, kind))
 
 
 def accuracy_f32(f, k, count):

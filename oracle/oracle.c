@@ -207,7 +207,8 @@ static inline uint16_t to_f16(float fin) {
   uint16_t o;
   f ^= sign;
   if (f >= (uint32_t)(127 + 16) << 23) {              /* overflow, inf, nan */
-    o = (f > 0x7f800000u) ? 0x7e00 : 0x7c00;
+    /* nan: quieted, payload kept (the conversion of gfx950 and IEEE 754) */
+    o = (f > 0x7f800000u) ? (uint16_t)(0x7e00u | ((f & 0x7fffffu) >> 13)) : 0x7c00;
   } else if (f < (uint32_t)113 << 23) {                /* half subnormal / 0 */
     const uint32_t magic = (uint32_t)((127 - 15) + (23 - 10) + 1) << 23;
     float r = u2f(f) + u2f(magic);
@@ -258,11 +259,12 @@ void oracle_run_lp(uint16_t *f1, uint16_t *f2, int nx, int ny, int nz,
   free(b);
 }
 
-/* Stochastic rounding to BF16 (no reference counterpart).  The hash is
- * "lowbias32" (C. Wellons, hash-prospector) of a Weyl-style combination of the
- * seed, the index of the cell's x pair and the step number. */
-uint32_t oracle_sr_hash(uint32_t seed, uint32_t pair, uint32_t step) {
-  uint32_t x = pair * 0x9E3779B1u + step * 0x85EBCA77u + seed;
+/* Stochastic rounding of 16-bit storage (no reference counterpart).  The
+ * random bits of a row and step come from the "lowbias32" integer mixer
+ * (C. Wellons, hash-prospector) of a Weyl-style combination of the seed, the
+ * row index and the step number; the cells of the row add a Weyl increment
+ * per x pair to it. */
+static inline uint32_t lowbias32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
   x ^= x >> 15;
@@ -271,31 +273,50 @@ uint32_t oracle_sr_hash(uint32_t seed, uint32_t pair, uint32_t step) {
   return x;
 }
 
+uint32_t oracle_sr_hash(uint32_t seed, uint32_t row, uint32_t pair, uint32_t step) {
+  return lowbias32(row * 0x9E3779B1u + step * 0x85EBCA77u + seed) + pair * 0x9E3779B9u;
+}
+
+uint16_t oracle_sr_round(float fin, uint32_t r, int kind) {
+  const uint32_t u = f2u(fin);
+  const uint32_t a = u & 0x7fffffffu;
+  if (kind == 0) {
+    if (a >= 0x7f800000u) return to_bf16(fin);           /* inf, nan: nearest */
+    return (uint16_t)((u + (r & 0xffffu)) >> 16);
+  }
+  const uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+  if (a >= 0x7f800000u) return to_f16(fin);              /* inf, nan: nearest */
+  const uint32_t rr = (r & 0xffffu) >> 3;                /* 13 bits: units of ulp / 2^13 */
+  const uint32_t e = a >> 23;
+  if (e >= 113u) {                                       /* FP16 normal range and above */
+    const uint32_t t = (a + rr) & 0xffffe000u;           /* an FP16 value, or >= 2^16 */
+    if (t >= 0x47800000u) return (uint16_t)(sign | 0x7c00u);
+    return (uint16_t)(sign | to_f16(u2f(t)));
+  }
+  /* |u| below 2^-14: in units of 2^-37 (= 2^-24 / 2^13) it is m * 2^(e-113);
+   * the FP16 subnormal count is floor((that + rr) / 2^13), and only the
+   * integer part of the scaled value can matter */
+  const uint32_t m = e ? ((a & 0x7fffffu) | 0x800000u) : (a & 0x7fffffu);
+  const uint32_t sh = 113u - (e ? e : 1u);
+  const uint32_t scaled = sh >= 32u ? 0u : m >> sh;
+  return (uint16_t)(sign | ((scaled + rr) >> 13));       /* 0x400: the least normal */
+}
+
 void oracle_run_lp_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF], int count,
                       int kind, uint32_t seed, int step0) {
   const long n = (long)nx * ny * nz;
   float *a = (float *)malloc(sizeof(float) * (size_t)n);
   float *b = (float *)malloc(sizeof(float) * (size_t)n);
   oracle_lp_to_f32(f1, a, n, kind);
-  const uint32_t half_nx = (uint32_t)(nx + 1) / 2u;
   for (int i = 0; i < count; ++i) {
     oracle_step_f32(a, b, nx, ny, nz, c);
     const uint32_t step = (uint32_t)(step0 + i);
     for (long j = 0; j < n; ++j) {
       const uint32_t x = (uint32_t)(j % nx), row = (uint32_t)(j / nx);
-      const uint32_t h = oracle_sr_hash(seed, row * half_nx + x / 2u, step);
-      const uint32_t r = (x & 1u) ? h >> 16 : h & 0xffffu;
-      const uint32_t u = f2u(b[j]);
-      if (kind == 0) {
-        uint16_t v;
-        if ((u & 0x7f800000u) == 0x7f800000u) v = to_bf16(b[j]);
-        else v = (uint16_t)((u + r) >> 16);
-        a[j] = from_bf16(v);
-      } else {
-        const uint32_t e = (u >> 23) & 0xffu;
-        const float t = u2f((u + (r & 0x1fffu)) & 0xffffe000u);
-        a[j] = from_f16(to_f16((e >= 113u && e <= 142u) ? t : b[j]));
-      }
+      const uint32_t h = oracle_sr_hash(seed, row, x / 2u, step);
+      const uint32_t r = (x & 1u) ? (h & 0xffffu) : (h >> 16);
+      const uint16_t v = oracle_sr_round(b[j], r, kind);
+      a[j] = kind == 0 ? from_bf16(v) : from_f16(v);
     }
   }
   oracle_f32_to_lp(a, f1, n, kind);

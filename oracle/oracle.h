@@ -56,20 +56,28 @@ void oracle_run_f64(double *f1, double *f2, int nx, int ny, int nz,
 void oracle_run_lp(uint16_t *f1, uint16_t *f2, int nx, int ny, int nz,
                    const float c[ORACLE_NCOEF], int count, int kind);
 void oracle_f32_to_lp(const float *in, uint16_t *out, long n, int kind);
-/* BF16 storage with deterministic stochastic rounding (no reference
- * counterpart): after step t (numbered from step0), the FP32 value u of cell
- * (x, y, z) gets 16 random bits added to its bit pattern before truncation to
- * BF16 (inf/nan round to nearest).  The bits are the low (x even) or high
- * (x odd) half of h = lowbias32(pair * 0x9E3779B1 + t * 0x85EBCA77 + seed),
- * pair = (z*ny + y) * ceil(nx/2) + x/2; oracle_sr_hash returns h. */
+/* 16-bit storage with deterministic stochastic rounding (no reference
+ * counterpart).  After step t (numbered from step0) the FP32 value u of cell
+ * (x, y, z) is rounded with 16 random bits r taken from
+ *   base = lowbias32(row * 0x9E3779B1 + t * 0x85EBCA77 + seed),  row = z*ny + y
+ *   h    = base + (x / 2) * 0x9E3779B9              (oracle_sr_hash)
+ * r = the high half of h for even x, the low half for odd x.
+ * kind 0 (BF16): finite u -> (bits(u) + r) >> 16, i.e. r added below the BF16
+ *   mantissa and truncated (can carry to inf); inf and nan round to nearest
+ *   (nan quieted).
+ * kind 1 (FP16): finite u -> |u| + (r >> 3) * ulp / 2^13 truncated to the FP16
+ *   grid (ulp = the FP16 ulp at |u|, 2^-24 below 2^-14, so subnormals round
+ *   stochastically too; results >= 2^16 become inf); inf and nan round to
+ *   nearest (nan quieted, payload kept).
+ * These are the rounding rules of gfx950's v_cvt_sr_bf16_f32 and
+ * v_cvt_sr_f16_f32 given the random operand h (even x) or h << 16 (odd x). */
 void oracle_run_bf16_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF],
                         int count, uint32_t seed, int step0);
-/* kind 0: as oracle_run_bf16_sr.  kind 1 (FP16 storage): with the same 16
- * bits r, values of FP32 exponent 113..142 (2^-14 <= |u| < 2^16) get r & 0x1fff
- * added below the FP16 mantissa before truncation; others round to nearest. */
 void oracle_run_lp_sr(uint16_t *f1, int nx, int ny, int nz, const float c[ORACLE_NCOEF], int count,
                       int kind, uint32_t seed, int step0);
-uint32_t oracle_sr_hash(uint32_t seed, uint32_t pair, uint32_t step);
+uint32_t oracle_sr_hash(uint32_t seed, uint32_t row, uint32_t pair, uint32_t step);
+/* One value rounded as above with the 16 random bits r (kind 0 BF16, 1 FP16). */
+uint16_t oracle_sr_round(float u, uint32_t r, int kind);
 void oracle_lp_to_f32(const uint16_t *in, float *out, long n, int kind);
 
 /* Legacy accuracy: RMS vs the analytical field with a float accumulator in

@@ -21,6 +21,17 @@ int main(void) {
     oracle_run_lp_sr(h, nx, ny, nz, k + 8, 5, kind, 7, 3); oracle_lp_to_f32(h, b, n, kind);
   }
   oracle_run_bf16_sr(h, nx, ny, nz, k + 8, 3, 1, 0);
+  /* the rounding rules at their edges: zeros, subnormals, overflow, inf, nan */
+  const uint32_t edge[] = {0u, 0x80000000u, 1u, 0x007fffffu, 0x387fffffu, 0x33000000u, 0x477ff000u,
+                           0x7f7fffffu, 0x7f800000u, 0xff800000u, 0x7fc00001u, 0xffbfffffu};
+  unsigned acc = 0;
+  for (int i = 0; i < (int)(sizeof edge / sizeof edge[0]); ++i) {
+    union { uint32_t u; float f; } v = {edge[i]};
+    for (int kind = 0; kind < 2; ++kind)
+      acc += oracle_sr_round(v.f, 0xffffu, kind) + oracle_sr_round(v.f, 0u, kind) +
+             oracle_sr_hash(9u, (uint32_t)i, 0xffffffffu, 0xffffffffu);
+  }
+  printf("%u\n", acc);
   printf("%g %g %g\n", oracle_accuracy_f32(a, nx, ny, nz, k, 7),
          oracle_rms_vs_analytical_f32(a, nx, ny, nz, k, 7), oracle_rms_vs_analytical_f64(ad, nx, ny, nz, kd, 7));
   double o[3]; oracle_diff_norms_f32(a, b, n, o); oracle_diff_norms_f64(ad, bd, n, o);

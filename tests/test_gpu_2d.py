@@ -209,6 +209,52 @@ def test_auto_choice_is_row_streaming_and_exact(ctx):
         assert np.array_equal(out.view(np.uint32), oracle.run2d_f32(f0, k[8:15], 13).view(np.uint32))
 
 
+def test_tuned_2d_entry_runs_the_2d_kernel(tmp_path):
+    """Tune-table entries record whether they name a 2D entry: the 2D entry
+    (vec 4, 2 groups, 4 rows, 1 wave, prefetch 2) has exactly the parameters
+    of a 3D tile, and a table naming it must run stencil2d_kernel, while the
+    same parameters flagged 3D run that 3D tile.  An autotuned nz == 1 field
+    runs the tile the tuner reported, through a save/load round trip."""
+    nx, ny = 2048, 1024
+    k = d3.constants2d_f32(nx, ny)
+    f0 = d3.initialize2d_f32(nx, ny, k)
+    c = d3.coefficients(k)
+    want = oracle.run2d_f32(f0, k[8:15], 9)
+
+    def twin(rows):
+        return [t for t in d3.tiles() if t["dtype"] == d3.F32 and t["temporal_depth"] == 1
+                and t["arith"] == d3.ARITH_EXACT and t["tile_x"] == 512 and t["groups"] == 2
+                and t["rows_per_wave"] == 4 and t["waves"] == 1 and t["prefetch"] == 2
+                and t["row_stream"] == rows]
+
+    t2d, t3d = twin(1), twin(0)
+    assert len(t2d) == 1 and len(t3d) == 1
+    for rows, tile in ((1, t2d[0]), (0, t3d[0])):
+        path = tmp_path / ("tune%d.txt" % rows)
+        # dtype depth fma log2nx log2ny log2nz vec G RPW NW pf zchunks ms depth batch rows
+        path.write_text("0 1 0 11 10 0 4 2 4 1 2 0 0.1 1 0 %d\n" % rows)
+        with d3.Context(0) as c2, d3.Field(c2, nx, ny, 1) as f:
+            c2.load_tuning(str(path))
+            f.set_y_reversed(True)
+            f.upload(f0)
+            tm = f.run(9, c, depth=1)
+            out = f.download().reshape(ny, nx)
+        assert tm["tile"] == tile["index"], (rows, tm)
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), rows
+    with d3.Context(0) as c3:
+        tuned = c3.autotune(nx, ny, 1, d3.F32, 1)
+        assert d3.tiles()[tuned["tile"]]["row_stream"] == 1, tuned
+        c3.save_tuning(str(tmp_path / "auto.txt"))
+    with d3.Context(0) as c4, d3.Field(c4, nx, ny, 1) as f:
+        c4.load_tuning(str(tmp_path / "auto.txt"))
+        f.set_y_reversed(True)
+        f.upload(f0)
+        tm = f.run(9, c, depth=1)
+        out = f.download().reshape(ny, nx)
+    assert tm["tile"] == tuned["tile"], (tm, tuned)
+    assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+
+
 def test_row_tiles_refuse_3d_operator(ctx):
     """A 2D tile is only for nz == 1 with ct == cb == 0; nz == 1 fields with
     t/b coefficients run on the 3D tiles and match the 7-point oracle."""

@@ -137,8 +137,10 @@ def test_batch_autotune_choices_keep_results(tmp_path):
             for i, p in enumerate(probs):
                 assert np.array_equal(bits(b.download(i)), bits(want[i])), (i, p)
         ctx.save_tuning(path)
+    # columns: dtype depth fma log2nx log2ny log2nz vec G RPW NW pf zchunks ms depth batch rows
     rows = [l.split() for l in open(path) if not l.startswith("#")]
-    assert any(r[-1] == "1" for r in rows)
+    assert any(r[14] == "1" for r in rows)
+    assert all(r[15] == "0" for r in rows if r[14] == "1")
     with d3.Context(0) as ctx2:
         ctx2.load_tuning(path)
         with d3.Batch(ctx2, probs) as b:

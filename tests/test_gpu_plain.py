@@ -1,9 +1,13 @@
 """GPU parity of the plain (one step per launch) kernels, through the C ABI.
 
-EXACT mode must be bitwise equal to the reference CPU path (fixtures from the
-reference's own build) and to the oracle; FMA mode must stay within 1e-6
-relative L2 (FP32).  FP64 is checked against the oracle's FP64 instantiation
-(bitwise in EXACT mode; the stated tolerance is 1e-13 relative L2).
+EXACT mode, the parity mode, must be bitwise equal to the reference CPU path
+(fixtures from the reference's own build) and to the oracle.  FMA mode is not
+a conforming mode: its drift from the reference CPU path (1.04e-6 relative L2
+after 100 steps at 128^3) exceeds the 1e-6 parity bound, so its test checks
+its own stated bound of 2e-6 per 100 steps at 128^3 and that it stays closer
+to the exact solution.  FP64 is checked against the oracle's FP64
+instantiation (bitwise in EXACT mode; the stated tolerance is 1e-13 relative
+L2).
 """
 import ctypes
 import hashlib
@@ -24,8 +28,9 @@ pytestmark = pytest.mark.gpu
 FULL_SHAPES = [(8, 8, 8), (16, 16, 16), (12, 10, 7), (33, 17, 5), (5, 9, 31)]
 # EXACT mode is the parity mode (bitwise).  FMA mode rounds differently from
 # the reference CPU build (whose g++ -O3 x86-64 code has no FMA); its drift
-# from that build is systematic (it is closer to the analytical solution), so
-# its stated bound vs the CPU path is per 100 steps at 128^3.
+# from that build is systematic (it is closer to the analytical solution) and
+# exceeds the 1e-6 parity bound, so FMA is a non-conforming mode with its own
+# stated bound of 2e-6 per 100 steps at 128^3 (not the parity bound).
 FP32_FMA_REL_L2_PER_100 = 2e-6
 FP64_REL_L2 = 1e-13
 
@@ -404,3 +409,33 @@ def test_contexts_on_concurrent_host_threads():
         want = oracle.run_f32(f0, k[8:15], 21)
         for out in outs:
             assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), i
+
+
+def test_field_copy_device_to_device(ctx):
+    """d3h_field_copy: the copy holds the source's current contents and step
+    count (stochastic rounding continues the source's step numbers), and
+    mismatched fields are refused."""
+    shape = (37, 23, 11)
+    k = d3.constants_f32(*shape)
+    f0 = d3.add_noise_f32(d3.initialize_f32(*shape, k), 21, 1e-3)
+    coef = d3.coefficients(k)
+    with d3.Field(ctx, *shape) as a, d3.Field(ctx, *shape) as b:
+        a.upload(f0)
+        a.run(3, coef)                       # odd: a's current buffer is its second
+        b.copy_from(a)
+        b.run(4, coef)
+        assert np.array_equal(b.download().view(np.uint32),
+                              oracle.run_f32(f0, k[8:15], 7).view(np.uint32))
+    bits = oracle.to_lp(f0, 0)
+    with d3.Field(ctx, *shape, d3.BF16) as a, d3.Field(ctx, *shape, d3.BF16) as b:
+        a.upload(bits, d3.BF16)
+        a.run(5, coef, rounding=d3.ROUND_STOCHASTIC, seed=3)
+        b.copy_from(a)
+        b.run(6, coef, rounding=d3.ROUND_STOCHASTIC, seed=3)
+        assert np.array_equal(b.download(), oracle.run_bf16_sr(bits, np.float32(coef), 11, 3))
+    with d3.Field(ctx, *shape) as a, d3.Field(ctx, 37, 23, 12) as b, \
+            d3.Field(ctx, *shape, d3.F64) as c:
+        for dst, src in ((b, a), (c, a)):
+            with pytest.raises(d3.D3HError) as e:
+                dst.copy_from(src)
+            assert e.value.status == N.D3H_ERR_INVALID

@@ -9,12 +9,13 @@
   shown to change the fixture bits, so the fixtures do pin the term order.
 """
 import hashlib
+import os
 
 import numpy as np
 import pytest
 
 import oracle
-from conftest import as_grid, shape_key
+from conftest import GOLDEN, as_grid, shape_key
 
 FULL_SHAPES = [(8, 8, 8), (16, 16, 16), (12, 10, 7), (33, 17, 5), (5, 9, 31)]
 
@@ -280,34 +281,68 @@ def _lowbias32_np(x):
     return x
 
 
-def _sr_hash_np(seed, pair, step):
+def _sr_hash_np(seed, row, pair, step):
+    """Independent restatement: lowbias32 of the row and step, plus a Weyl
+    increment per x pair."""
+    row = np.asarray(row, dtype=np.uint64)
     pair = np.asarray(pair, dtype=np.uint64)
-    return _lowbias32_np((pair * 0x9E3779B1 + step * 0x85EBCA77 + seed) & 0xFFFFFFFF)
+    base = _lowbias32_np((row * 0x9E3779B1 + step * 0x85EBCA77 + seed) & 0xFFFFFFFF)
+    return (base + pair * 0x9E3779B9) & 0xFFFFFFFF
+
+
+def _sr_bits_np(shape2d, step, seed):
+    """The 16 random bits of every cell of an (rows, nx) array."""
+    rows, nx = shape2d
+    x = np.arange(nx, dtype=np.uint64)
+    h = _sr_hash_np(seed, np.arange(rows, dtype=np.uint64)[:, None], x[None, :] // 2, step)
+    return np.where(x[None, :] & 1, h & 0xFFFF, h >> 16)
 
 
 def test_sr_hash_matches_independent_restatement():
     rng = np.random.default_rng(0)
-    for seed, cell, step in zip(rng.integers(0, 2**32, 50), rng.integers(0, 2**32, 50),
-                                rng.integers(0, 2**31, 50)):
-        assert oracle.sr_hash(int(seed), int(cell), int(step)) == int(
-            _sr_hash_np(int(seed), int(cell), int(step)))
-    # neighbouring pairs give unrelated 16-bit halves
-    h = np.array([oracle.sr_hash(7, c, 3) for c in range(4096)], np.uint64)
+    for seed, row, pair, step in zip(rng.integers(0, 2**32, 50), rng.integers(0, 2**32, 50),
+                                     rng.integers(0, 2**31, 50), rng.integers(0, 2**31, 50)):
+        assert oracle.sr_hash(int(seed), int(row), int(pair), int(step)) == int(
+            _sr_hash_np(int(seed), int(row), int(pair), int(step)))
+    # along a row, both halves take every value about equally often and
+    # neighbouring cells' bits are unrelated
+    h = np.array([oracle.sr_hash(7, 11, c, 3) for c in range(4096)], np.uint64)
     for half in (h & 0xFFFF, h >> 16):
         assert 0.45 < (half < 32768).mean() < 0.55 and len(set(half.tolist())) > 3900
     assert abs(np.corrcoef((h & 0xFFFF).astype(float), (h >> 16).astype(float))[0, 1]) < 0.05
+    # one cell over consecutive steps, and one column over consecutive rows
+    for seq in (np.array([oracle.sr_hash(7, 11, 5, t) for t in range(4096)], np.uint64),
+                np.array([oracle.sr_hash(7, y, 5, 3) for y in range(4096)], np.uint64)):
+        for half in (seq & 0xFFFF, seq >> 16):
+            v = half.astype(float)
+            assert 0.45 < (v < 32768).mean() < 0.55
+            assert abs(np.corrcoef(v[:-1], v[1:])[0, 1]) < 0.05
 
 
 def _sr_bf16_np(f, step, seed):
-    """Stochastic rounding of an (nz, ny, nx) float32 array (or a 1-D row)."""
-    f = np.atleast_3d(f) if f.ndim != 3 else f
+    """BF16 stochastic rounding of an (nz, ny, nx) float32 array (finite
+    values): the cell's 16 random bits added below the mantissa, truncated."""
     nx = f.shape[-1]
     u = f.view(np.uint32).reshape(-1, nx).astype(np.uint64)
-    x = np.arange(nx, dtype=np.uint64)
-    pair = np.arange(u.shape[0], dtype=np.uint64)[:, None] * ((nx + 1) // 2) + x[None, :] // 2
-    h = _sr_hash_np(seed, pair, step)
-    r = np.where(x[None, :] & 1, h >> 16, h & 0xFFFF)
+    r = _sr_bits_np(u.shape, step, seed)
     return (((u + r) >> 16) & 0xFFFF).astype(np.uint16).reshape(f.shape)
+
+
+def test_sr_round_matches_gfx950_conversions():
+    """oracle_sr_round against outputs of gfx950's v_cvt_sr_bf16_f32 and
+    v_cvt_sr_f16_f32 recorded on an MI355X (tests/golden/gen_sr_hw.py): the
+    instruction takes the high 16 bits of its random operand, so cell bits r
+    are the operand r << 16.  Zeros, subnormals, overflow edges, inf and nan
+    payloads included."""
+    v = np.load(os.path.join(GOLDEN, "sr_hw_vectors.npz"))
+    x, r = v["x"], v["r"]
+    assert len(x) > 18000
+    fx = x.view(np.float32)
+    for kind, want in ((0, v["bf16"]), (1, v["f16"])):
+        got = np.array([oracle.sr_round(fx[i], int(r[i] >> 16), kind) for i in range(len(x))],
+                       np.uint16)
+        bad = np.nonzero(got != want)[0]
+        assert len(bad) == 0, [(hex(x[i]), hex(r[i]), hex(got[i]), hex(want[i])) for i in bad[:5]]
 
 
 def test_bf16_sr_run_matches_numpy_restatement():
@@ -338,11 +373,18 @@ def test_bf16_sr_rounds_to_a_neighbour_without_bias():
     # exactly representable values are kept
     xr = oracle.from_lp(oracle.to_lp(x, 0), 0)
     assert np.array_equal(oracle.from_lp(_sr_bf16_np(xr, 1, 5), 0), xr)
-    # unbiased: one value rounded at many cells averages back to itself
-    v = np.full((1, 1, 400000), np.float32(0.1234567), np.float32)
-    m = oracle.from_lp(_sr_bf16_np(v, 2, 9), 0).astype(np.float64).mean()
+    # unbiased: one value rounded at many cells (one row and many rows)
+    # averages back to itself
     ulp = 2.0 ** -11                                                 # bf16 ulp at 0.12
-    assert abs(m - 0.1234567) < 5 * ulp / np.sqrt(v.size)
+    for shape in ((1, 1, 400000), (1, 400000, 1)):
+        v = np.full(shape, np.float32(0.1234567), np.float32)
+        m = oracle.from_lp(_sr_bf16_np(v, 2, 9), 0).astype(np.float64).mean()
+        assert abs(m - 0.1234567) < 5 * ulp / np.sqrt(v.size), shape
+    # and one cell rounded at many steps averages back to itself
+    r = np.array([oracle.sr_hash(9, 4, 3, t) >> 16 for t in range(100000)], np.uint64)
+    u = np.uint64(np.float32(0.1234567).view(np.uint32))
+    m = ((((u + r) >> 16) << 16).astype(np.uint32).view(np.float32)).astype(np.float64).mean()
+    assert abs(m - 0.1234567) < 5 * ulp / np.sqrt(r.size)
 
 
 def test_bf16_sr_tracks_fp64_where_nearest_stalls():
@@ -390,24 +432,25 @@ def test_fp64_constants_and_field_closed_form(shape):
 
 
 def _sr_f16_np(f, step, seed):
-    """FP16 stochastic rounding of an (nz, ny, nx) float32 array -> float32
-    values (numpy restatement)."""
+    """FP16 stochastic rounding of an (nz, ny, nx) float32 array (finite
+    values) -> float32 values, in float64 arithmetic: |x| + (r >> 3) ulp / 2^13
+    truncated to the FP16 grid (ulp 2^-24 below 2^-14), sign kept, >= 2^16 to
+    inf."""
     nx = f.shape[-1]
-    u = f.view(np.uint32).reshape(-1, nx).astype(np.uint64)
-    x = np.arange(nx, dtype=np.uint64)
-    pair = np.arange(u.shape[0], dtype=np.uint64)[:, None] * ((nx + 1) // 2) + x[None, :] // 2
-    h = _sr_hash_np(seed, pair, step)
-    r = np.where(x[None, :] & 1, h >> 16, h & 0xFFFF) & 0x1FFF
-    t = (((u + r) & 0xFFFFE000) & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
-    e = (u >> 23) & 0xFF
-    src = np.where((e >= 113) & (e <= 142), t, f.reshape(-1, nx))
-    return src.astype(np.float16).astype(np.float32).reshape(f.shape)
+    u = f.view(np.uint32).reshape(-1, nx)
+    rr = (_sr_bits_np(u.shape, step, seed) >> 3).astype(np.float64)
+    a = np.abs(f.reshape(-1, nx).astype(np.float64))
+    e = ((u >> 23) & 0xFF).astype(np.int64)
+    ulp = np.where(e < 113, 2.0 ** -24, np.ldexp(1.0, np.maximum(e, 113) - 137))
+    q = np.floor((a + rr * ulp / 8192.0) / ulp) * ulp
+    q = np.where(q >= 65536.0, np.inf, q)
+    return np.copysign(q, f.reshape(-1, nx).astype(np.float64)).astype(np.float32).reshape(f.shape)
 
 
 def test_f16_sr_run_matches_numpy_restatement():
     rng = np.random.default_rng(8)
     f0 = (rng.random((6, 9, 7), dtype=np.float32) * np.float32(0.5)).astype(np.float32)
-    f0[0, 0, :3] = [0.0, 3e-6, -2e-7]            # zero and FP16 subnormals: nearest even
+    f0[0, 0, :3] = [0.0, 3e-6, -2e-7]            # zero and FP16 subnormals (stochastic too)
     bits = oracle.to_lp(f0, 1)
     c = np.array([0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4], np.float32)
     a = oracle.from_lp(bits, 1)
