@@ -95,6 +95,8 @@ SIGNATURES = {
     "d3h_field_compare": (c_int, [c_void_p, c_void_p, P(c_double)]),
     "d3h_field_error_analytical": (c_int, [c_void_p, c_double, c_double, P(c_double), P(c_double),
                                            P(c_double)]),
+    "d3h_field_error_analytical_noalloc": (c_int, [c_void_p, c_double, c_double, P(c_double), P(c_double),
+                                           P(c_double)]),
     "d3h_tile_count": (c_int, []),
     "d3h_tile_info": (c_int, [c_int, P(TileDesc)]),
     "d3h_autotune": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, P(c_int), P(c_int),

@@ -432,6 +432,70 @@ static int ensure_partials(d3h_context *ctx) {
   return D3H_OK;
 }
 
+// ---- allocation-free analytical error norms -------------------------------------
+// d3h_field_error_analytical_noalloc keeps everything it needs beyond the
+// field and the caller's out[2] in this file-scope device array: the per-axis
+// factors 1 - a cos(k x) (nx + ny + nz <= kNoAllocTab), the per-block partial
+// sums and the two results.  Each device holds its own copy until the process
+// ends; its address is looked up once per device by d3h_context_create
+// (hipGetSymbolAddress, with the device current) and kept in g_scratch_dev.
+constexpr int kNoAllocTab = 1 << 16;
+constexpr int kNoAllocBlocks = 2048;
+constexpr int kNoAllocDoubles = kNoAllocTab + 2 * kNoAllocBlocks + 2;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices];
+
+// Once per device, with that device current.
+static int init_noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return D3H_OK;  // the entry point refuses such devices
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return D3H_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(D3H_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_dev[device] = static_cast<double *>(ptr);
+  return D3H_OK;
+}
+
+// tab[i] for i in [0, nx + ny + nz): 1 - amp[a] cos(k[a] (d[a] (j + 0.5))) of
+// axis a and index j, as the host computes it for d3h_field_error_analytical.
+__global__ void __launch_bounds__(256)
+axis_factor_kernel(double *tab, int nx, int ny, int nz, double ax, double ay, double az, double kx,
+                   double ky, double kz, double dx, double dy, double dz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx + ny + nz) return;
+  double amp, k, d;
+  int j;
+  if (i < nx) {
+    amp = ax; k = kx; d = dx; j = i;
+  } else if (i < nx + ny) {
+    amp = ay; k = ky; d = dy; j = i - nx;
+  } else {
+    amp = az; k = kz; d = dz; j = i - nx - ny;
+  }
+  tab[i] = 1.0 - amp * cos(k * (d * (j + 0.5)));
+}
+
+// One workgroup: out[0] = sqrt(sum of part[2i] / n), out[1] = max of part[2i+1].
+__global__ void __launch_bounds__(kRedThreads)
+rms_max_final_kernel(const double *part, int nb, double n, double *out) {
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) {
+    v[0] += part[2 * i];
+    v[1] = fmax(v[1], part[2 * i + 1]);
+  }
+  __shared__ double fin[2];
+  block_reduce<2>(v, fin, true);  // blockIdx.x == 0: writes fin[0], fin[1]
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(fin[0] / n);
+    out[1] = fin[1];
+  }
+}
+
 // ---- HBM probe ----------------------------------------------------------------
 // One 16-byte vector per thread, non-temporal load and store: the fastest
 // streaming form measured on MI355X (the reference's "copy" variant computes
@@ -506,6 +570,13 @@ int d3h_context_create(int device, d3h_context **out) {
   }
   c->num_cus = prop.multiProcessorCount;
   c->lds_per_block = (int)prop.sharedMemPerBlock;
+  {
+    const int rc = init_noalloc_scratch(device);
+    if (rc != D3H_OK) {
+      delete c;
+      return rc;
+    }
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -921,6 +992,40 @@ int d3h_field_error_analytical(const d3h_field *f, double kappa, double time, co
   }
   out[0] = std::sqrt(d2 / ((double)f->nx * f->ny * f->nz));
   out[1] = mx;
+  return D3H_OK;
+}
+
+int d3h_field_error_analytical_noalloc(const d3h_field *f, double kappa, double time, const double k[3],
+                                       const double d[3], double out[2]) {
+  if (!f || !k || !d || !out) return fail(D3H_ERR_INVALID, "NULL argument");
+  if ((long long)f->nx + f->ny + f->nz > kNoAllocTab)
+    return fail(D3H_ERR_INVALID, "nx + ny + nz = %lld exceeds the %d factors of the static scratch",
+                (long long)f->nx + f->ny + f->nz, kNoAllocTab);
+  d3h_context *ctx = f->ctx;
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return fail(D3H_ERR_UNSUPPORTED, "device %d: the static scratch serves devices 0-%d", ctx->device,
+                kMaxDevices - 1);
+  D3H_HIP(hipSetDevice(ctx->device));
+  double *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_dev[ctx->device];
+  }
+  if (!base) return fail(D3H_ERR_STATE, "static scratch of device %d not set up", ctx->device);
+  double *tab = base, *part = base + kNoAllocTab, *res = part + 2 * kNoAllocBlocks;
+  const int ntab = f->nx + f->ny + f->nz;
+  double amp[3];
+  for (int a = 0; a < 3; ++a) amp[a] = std::exp(-kappa * time * k[a] * k[a]);
+  hipLaunchKernelGGL(axis_factor_kernel, dim3((ntab + 255) / 256), dim3(256), 0, ctx->stream, tab, f->nx,
+                     f->ny, f->nz, amp[0], amp[1], amp[2], k[0], k[1], k[2], d[0], d[1], d[2]);
+  const int nb = std::min(std::max(1, ctx->num_cus) * 4, kNoAllocBlocks);
+  hipLaunchKernelGGL(analytical_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream, f->buf[f->cur],
+                     f->dtype, f->pitch, f->nx, f->ny, (long long)f->ny * f->nz, (const double *)tab, part);
+  hipLaunchKernelGGL(rms_max_final_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, (const double *)part,
+                     nb, (double)f->nx * f->ny * f->nz, res);
+  D3H_HIP(hipGetLastError());
+  D3H_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  D3H_HIP(hipStreamSynchronize(ctx->stream));
   return D3H_OK;
 }
 

@@ -268,6 +268,16 @@ class Field:
               "d3h_field_error_analytical")
         return {"rms": out[0], "max_abs": out[1]}
 
+    def error_analytical_noalloc(self, kappa, time, k, d):
+        """error_analytical without any allocation (a static device scratch
+        shared by every call on the device: calls must not overlap)."""
+        out = (ctypes.c_double * 2)()
+        kk = (ctypes.c_double * 3)(*k)
+        dd = (ctypes.c_double * 3)(*d)
+        check(lib().d3h_field_error_analytical_noalloc(self.handle, kappa, time, kk, dd, out),
+              "d3h_field_error_analytical_noalloc")
+        return {"rms": out[0], "max_abs": out[1]}
+
 
 def run_kernel2d(count, f1, coef):
     """The diffusion/ family's 2D RunKernel(count) (diffusion/diffusion_openmp.cc:59-82)

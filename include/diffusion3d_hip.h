@@ -193,6 +193,21 @@ int d3h_field_compare(const d3h_field *a, const d3h_field *b, double out[3]);
  * (diffusion3d/diffusion3d.cc:14-39).  out[0] = RMS, out[1] = max abs. */
 int d3h_field_error_analytical(const d3h_field *f, double kappa, double time,
                                const double k[3], const double d[3], double out[2]);
+/* The same norms, computed without allocating anything: no counterpart in
+ * the reference; for callers that cannot allocate device or pinned memory
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  d3h_field_error_analytical allocates its per-axis table on
+ * every call and its reduction scratch on a context's first call; this
+ * variant keeps the table, the per-workgroup partial sums and the results in
+ * a static device array of 557 KB per device (set up by d3h_context_create)
+ * and reduces on the device, so it calls no hipMalloc/hipFree.  Limit:
+ * nx + ny + nz <= 65536 (D3H_ERR_INVALID beyond), devices 0-63.  Every call
+ * on one device shares that device's array: calls that overlap on one device
+ * (other contexts or host threads) race and must be serialised by the
+ * caller.  Results equal d3h_field_error_analytical's to rounding of the FP64
+ * reductions (the factors are evaluated with the device's cos). */
+int d3h_field_error_analytical_noalloc(const d3h_field *f, double kappa, double time,
+                                       const double k[3], const double d[3], double out[2]);
 
 /* Kernel/tile table (the tuning space of diffusion3d/build.py:6-34, searched
  * at run time).  Returns D3H_ERR_NOT_FOUND past the end. */
