@@ -110,18 +110,28 @@ def cpu_baseline_2d(nx, ny, steps_sample):
                       % (nx, ny, steps_sample, seconds)}
 
 
-def cpu_baseline(nx, ny, nz, steps_sample):
-    """Time the reference's OpenMP RunKernel (or the oracle port) on the host."""
+def cpu_baseline(nx, ny, nz, steps_sample, temporal_blocking=False):
+    """Time the reference's OpenMP RunKernel (or the oracle port) on the host.
+    temporal_blocking: the reference's Diffusion3DOpenMPTemporalBlocking
+    (diffusion3d/diffusion3d_openmp_temporal_blocking.cc:8-68), which needs
+    nz % threads == 0 and an even nz / threads and runs count + 1 steps for odd
+    counts, so the sample is an even step count; None when the thread count
+    does not divide the grid that way."""
     gomp = ctypes.CDLL("libgomp.so.1")
     gomp.omp_get_max_threads.restype = ctypes.c_int
     threads = gomp.omp_get_max_threads()
     ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    if temporal_blocking:
+        if not os.path.exists(ref) or nz % threads or (nz // threads) % 2:
+            return None
+        steps_sample += steps_sample % 2
     if os.path.exists(ref):
         L = ctypes.CDLL(ref)
         fp = ctypes.POINTER(ctypes.c_float)
         L.ref_run.argtypes = [ctypes.c_int] * 5 + [fp, fp, fp, fp, ctypes.POINTER(ctypes.c_double)]
         sec = ctypes.c_double()
-        rc = L.ref_run(1, nx, ny, nz, steps_sample, None, None, None, None, ctypes.byref(sec))
+        rc = L.ref_run(2 if temporal_blocking else 1, nx, ny, nz, steps_sample, None, None, None,
+                       None, ctypes.byref(sec))
         if rc != 0:
             raise RuntimeError("ref_run failed")
         seconds, kind = sec.value, "reference"
@@ -139,8 +149,9 @@ def cpu_baseline(nx, ny, nz, steps_sample):
     cells = nx * ny * nz * steps_sample
     return {"value": cells / seconds / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": kind,
-            "sample": "%dx%dx%d FP32, %d steps, OpenMP RunKernel wall clock (%.2f s)"
-                      % (nx, ny, nz, steps_sample, seconds)}
+            "sample": "%dx%dx%d FP32, %d steps, %s RunKernel wall clock (%.2f s)"
+                      % (nx, ny, nz, steps_sample,
+                         "OpenMP temporal-blocking" if temporal_blocking else "OpenMP", seconds)}
 
 
 def depth_desc(depth):
@@ -428,13 +439,6 @@ def main():
     info = ctx.info()
     print("device: %d CUs, %.1f GB HBM" % (info["num_cus"], info["total_bytes"] / 1e9),
           file=sys.stderr, flush=True)
-    probe = ctx.bandwidth_probe(1 << 28, d3.F32, reps=10)
-    print("copy probe: %.1f GB/s" % probe["gbps"], file=sys.stderr, flush=True)
-    tuned = None
-    if args.tune:
-        tuned = ctx.autotune(nx, ny, nz, dtype, depth, arith)
-        print("tuned:", tuned, file=sys.stderr, flush=True)
-
     two_d = nz == 1
     if two_d:
         # the diffusion/ family's 2D benchmark: rows reversed on the device so
@@ -464,6 +468,15 @@ def main():
         field.set_y_reversed(True)
         pristine.set_y_reversed(True)
     pristine.upload(initial())
+    # the probe and the autotuner run after the host-side initialisation, so
+    # the device stays busy from them through the warm-up to the timed run
+    probe = ctx.bandwidth_probe(1 << 28, d3.F32, reps=10)
+    print("copy probe: %.1f GB/s" % probe["gbps"], file=sys.stderr, flush=True)
+    tuned = None
+    if args.tune:
+        tuned = ctx.autotune(nx, ny, nz, dtype, depth, arith)
+        print("tuned:", tuned, file=sys.stderr, flush=True)
+
     field.copy_from(pristine)
     if args.warmup > 0:
         field.run(args.warmup, coef, depth=depth, arith=arith, graph=False, **rnd)
@@ -531,6 +544,8 @@ def main():
         if prof.get("kernel") == name and prof.get("grid_size", grid) == grid:
             traffic = prof.get("hbm_bytes_per_launch")
 
+    depth_used = tm["temporal_depth"]
+    cpu_plain = None
     served = None
     if args.workload == "plain512_f32" and args.served_requests > 0:
         served = served_summary(d3, ctx, args)
@@ -541,9 +556,19 @@ def main():
         if two_d:
             cpu = cpu_baseline_2d(nx, ny, max(1, args.cpu_steps * CPU_SAMPLE_CELLS // (nx * ny)))
         else:
-            cpu = cpu_baseline(nx, ny, nz, args.cpu_steps)
+            # a sample of about --cpu-steps 512^3 steps' worth of cell updates
+            sample = max(2, args.cpu_steps * CPU_SAMPLE_CELLS // (nx * ny * nz))
+            cpu = cpu_baseline(nx, ny, nz, sample)
+            if depth_used > 1:
+                # the reference's own CPU temporal blocking beside its plain
+                # OpenMP path (the GPU counterpart of the fused launches)
+                # (it runs several times slower than the plain OpenMP path:
+                # per-iteration work-slab allocation and the extra planes, so
+                # it gets a quarter of the sample)
+                tb = cpu_baseline(nx, ny, nz, max(2, sample // 4), temporal_blocking=True)
+                if tb is not None:
+                    cpu, cpu_plain = tb, cpu
 
-    depth_used = tm["temporal_depth"]
     size = ("%d^2 (2D 5-point)" % nx) if two_d else ("%d^3" % nx)
     if dtype == d3.BF16:
         size += " (BF16 storage)"
@@ -577,6 +602,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "cpu_baseline_plain_openmp": cpu_plain,
         "kernel_ms": round(tm["kernel_ms"], 3),
         "avg_launch_ms": round(tm["avg_launch_ms"], 5),
         "hbm_gbps_wall": round(achieved_wall, 1),

@@ -229,8 +229,9 @@ int d3h_tile_info(int index, d3h_tile_desc *out);
 /* Autotuner: time every matching tile (and z-chunk count) on a scratch field of
  * the given shape, remember the fastest for that size class; the table can be
  * saved to / loaded from a text file.  best_tile / best_zchunks may be NULL.
- * temporal_depth 0 searches depths 1, 2, 4 and 8 and also remembers the
- * fastest depth for the class, used by runs, batches and servers given depth
+ * temporal_depth 0 searches depths 1, 2 and 4 (and 8 for nz == 1 fields,
+ * whose 2D kernels gain from it; 3D depth-8 tiles measured slower than plain
+ * steps) and also remembers the fastest depth for the class, used by runs, batches and servers given depth
  * 0; best_tile then names the chosen depth's tile (d3h_tile_info).  Every
  * candidate is timed twice, interleaved with the others, and keeps its
  * faster time; BF16/FP16 storage is timed with stochastic rounding. */
