@@ -20,7 +20,9 @@ namespace d3h {
   D3H_ENTRY_PF(kF32, 4, 1, 2, 1, 4, A, 2), D3H_ENTRY_PF(kF32, 4, 1, 2, 1, 4, A, 3), \
   D3H_ENTRY_PF(kF32, 4, 1, 2, 2, 4, A, 2), D3H_ENTRY_PF(kF32, 4, 1, 2, 1, 8, A, 2), \
   D3H_ENTRY_PF(kF32, 4, 1, 2, 2, 2, A, 2), D3H_ENTRY_PF(kF32, 4, 1, 3, 1, 4, A, 2), \
-  D3H_ENTRY_PF(kF32, 4, 1, 2, 4, 1, A, 2)
+  D3H_ENTRY_PF(kF32, 4, 1, 2, 4, 1, A, 2),                                         \
+  /* one cell per lane: 64-wide tiles for narrow (batched) grids */              \
+  D3H_ENTRY_B(kF32, 1, 1, 1, 4, 4, A), D3H_ENTRY_B(kF32, 1, 1, 1, 8, 2, A)
 
 static const KernelEntry kTable[] = {F32_PLAIN(false), F32_PLAIN(true)};
 

@@ -13,7 +13,9 @@ namespace d3h {
   D3H_ENTRY_B(kF32, 2, 2, 1, 4, 8, A), D3H_ENTRY(kF32, 4, 2, 2, 2, 4, A),           \
   D3H_ENTRY(kF32, 2, 4, 1, 3, 8, A), D3H_ENTRY_B(kF32, 2, 4, 1, 2, 16, A), D3H_ENTRY(kF32, 4, 4, 1, 2, 8, A),          \
   D3H_ENTRY(kF32, 2, 4, 1, 4, 8, A), D3H_ENTRY(kF32, 4, 2, 3, 2, 8, A),            \
-  D3H_ENTRY(kF32, 2, 8, 1, 3, 8, A)
+  D3H_ENTRY(kF32, 2, 8, 1, 3, 8, A),                                               \
+  /* one cell per lane: 64-wide tiles for narrow (batched) grids */              \
+  D3H_ENTRY_B(kF32, 1, 2, 1, 4, 8, A), D3H_ENTRY_B(kF32, 1, 4, 1, 4, 8, A)
 
 static const KernelEntry kTable[] = {F32_TB(false), F32_TB(true)};
 

@@ -5,10 +5,12 @@
 //
 // Work decomposition
 //   * A workgroup owns an xy tile and a z-chunk.  The tile is TILE_X = 64 lanes
-//     x G groups x VEC cells wide (VEC cells = one 16- or 8-byte vector) and
+//     x G groups x VEC cells wide (VEC cells = one 16-, 8- or 4-byte vector) and
 //     TILE_Y = NW waves x RPW rows tall.  Lane l of group g holds cells
 //     x0 + g*64*VEC + l*VEC + [0, VEC) of each of its wave's RPW consecutive
-//     rows, so every HBM access is a coalesced 128-bit (or 64-bit) access.
+//     rows, so every HBM access is a coalesced 128-bit (64-, 32-bit) access; the
+//     4-byte vectors make 64-wide FP32 and 128-wide 16-bit tiles for narrow
+//     grids.
 //   * The workgroup streams z: iteration i requests plane i + PF (PF = 1..3
 //     planes in flight) into registers.  Stage s (1..T) then computes time
 //     level s at plane i - s from level s-1 planes (i-s-1, i-s, i-s+1), all
@@ -124,7 +126,7 @@ struct Pack {
   using C = typename Traits<S>::C;
   static constexpr int kBytes = VEC * (int)sizeof(typename Traits<S>::T);
   static constexpr int kWords = kBytes / 4;
-  static_assert(kBytes == 8 || kBytes == 16, "lane vectors are 8 or 16 bytes");
+  static_assert(kBytes == 4 || kBytes == 8 || kBytes == 16, "lane vectors are 4, 8 or 16 bytes");
 
   __device__ static __forceinline__ void unpack(const uint32_t *w, C *v) {
     if constexpr (S == kF32) {
@@ -174,9 +176,11 @@ struct Pack {
     if constexpr (kBytes == 16) {
       const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, kLoadAux);
       w[0] = u.x; w[1] = u.y; w[2] = u.z; w[3] = u.w;
-    } else {
+    } else if constexpr (kBytes == 8) {
       const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, kLoadAux);
       w[0] = u.x; w[1] = u.y;
+    } else {
+      w[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, kLoadAux);
     }
     unpack(w, v);
   }
@@ -191,9 +195,11 @@ struct Pack {
     if constexpr (kBytes == 16) {
       u32x4 u = {w[0], w[1], w[2], w[3]};
       __builtin_amdgcn_raw_buffer_store_b128(u, rs, off, 0, kStoreAux);
-    } else {
+    } else if constexpr (kBytes == 8) {
       u32x2 u = {w[0], w[1]};
       __builtin_amdgcn_raw_buffer_store_b64(u, rs, off, 0, kStoreAux);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(w[0], rs, off, 0, kStoreAux);
     }
   }
 };
