@@ -330,6 +330,13 @@ struct XchLayout {
 // is computed.  Each level keeps NS = PF + 3 register slots (plane q in slot
 // q mod NS) and the z loop is unrolled by NS.
 //
+// PF = 0 (fused depths T >= 2 only) is the three-slot ring: every level,
+// level 0 included, keeps just the 3 planes its next stage reads, and plane
+// i + 1 is requested right after stage 1 of iteration i, into the slot of
+// plane i - 2 that stage 1 has just read for the last time; stages 2..T then
+// hide its latency.  That frees T * RPW * NV registers per lane against
+// PF = 1, enough for taller tiles at the same occupancy.
+//
 // W2D (one wave, NW = 1): the 2D 5-point operator on an nx x ny field whose
 // "planes" are blocks of RPW consecutive rows.  Row r's y neighbours are rows
 // r +- 1 of the same plane, or, across a plane boundary, the last row of the
@@ -347,6 +354,7 @@ struct TileRunner {
   static constexpr int TILE_X = G * GSTRIDE;
   static constexpr int TILE_Y = NW * RPW;
   static constexpr int NS = PF + 3;
+  static_assert(PF > 0 || (T >= 2 && !W2D), "the three-slot ring needs fused depths");
   // A tile taller than its 2T halo rows has halo rows whose last-stage values
   // are never stored; the last stage skips their arithmetic.  (Skipping, at
   // stages 1..T-1, the rows those stages need not compute either was measured
@@ -656,17 +664,24 @@ struct TileRunner {
   }
 
   // Iteration i (i == PH mod NS): request plane i + PF, exchange the wave
-  // edge rows, run the stages.  The exchange buffer alternates with i.
+  // edge rows, run the stages (PF = 0: request plane i + 1 after stage 1).
+  // The exchange buffer alternates with i.
   template <int PH>
   __device__ __forceinline__ void iter(int i) {
-    load_plane<modn<NS>(PH + PF)>(i + PF);
+    if constexpr (PF > 0) load_plane<modn<NS>(PH + PF)>(i + PF);
     const int buf = (X::kBufs == 2) ? (i & 1) : 0;
     if constexpr (NW > 1) {
       if constexpr (X::kBufs == 1) __syncthreads();  // previous plane's reads are done
       publish<PH>(buf);
       __syncthreads();
     }
-    stages<PH, 1>(i, buf);
+    if constexpr (PF > 0) {
+      stages<PH, 1>(i, buf);
+    } else {
+      stage<PH, 1>(i, buf);
+      load_plane<modn<NS>(PH + 1)>(i + 1);  // the slot of plane i - 2, read last by stage 1
+      stages<PH, 2>(i, buf);
+    }
   }
 
   template <int PH>
@@ -676,9 +691,10 @@ struct TileRunner {
   }
 
   __device__ __forceinline__ void run() {
-    // the first plane (zl0) is requested at iteration zl0 - PF; start at the
-    // multiple of NS at or below it so phases are compile-time
-    const int a = zl0 - PF;
+    // the first plane (zl0) is requested at iteration zl0 - PF (PF = 0: at
+    // iteration zl0 - 1, after its first stage); start at the multiple of NS
+    // at or below it so phases are compile-time
+    const int a = zl0 - (PF > 0 ? PF : 1);
     const int i0 = a - modn<NS>(a);
     const int ilast = zo1 - 1 + T;
     for (int i = i0; i <= ilast; i += NS) iters<0>(i);

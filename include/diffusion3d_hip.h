@@ -217,7 +217,10 @@ typedef struct {
   int rows_per_wave;   /* rows each wavefront owns                          */
   int waves;           /* wavefronts per workgroup                          */
   int tile_x, tile_y;  /* loaded tile extent in cells / rows               */
-  int prefetch;        /* planes requested ahead of the one being computed  */
+  int prefetch;        /* planes requested ahead of the one being computed;
+                          0: the three-slot ring of fused depths (the next
+                          plane requested after the first stage, every
+                          level keeping 3 planes)                           */
   int row_stream;      /* 1: 2D kernel, streams the rows of nz == 1 fields in
                           blocks of rows_per_wave rows; such a
                           tile runs only fields with nz == 1 and ct == cb == 0,
