@@ -111,6 +111,15 @@ int make_geometry(const KernelEntry &e, int nx, int ny, int nz, int pitch, int z
   }
   nzc = std::max(1, std::min(nzc, nz));
   q.oz = (nz + nzc - 1) / nzc;
+  if (nzc > 1) {
+    // A chunk runs its z loop in whole groups of NS iterations (NS = the
+    // register-slot count, PF + 3, or 3 for the ring), oz + 2T + max(PF, 1)
+    // of them for an interior chunk: round oz up so that is a multiple of NS
+    // and no interior chunk computes planes past its end.
+    const int ns = e.pf > 0 ? e.pf + 3 : 3;
+    const int lead = 2 * T + std::max(e.pf, 1);
+    while ((q.oz + lead) % ns != 0) ++q.oz;
+  }
   q.nzc = (nz + q.oz - 1) / q.oz;
   const long long nb = tiles * q.nzc;
   if (nb <= 0 || nb > INT_MAX) return fail(D3H_ERR_UNSUPPORTED, "grid too large");

@@ -233,17 +233,30 @@ __device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
 
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
+// A value the program never reads, for the results of rows whose stores are
+// dropped: an empty asm statement defines it, so no instruction sets the
+// register.  The operand u (a value live at that point) keeps each statement
+// distinct, so they are not merged into one value copied around.
+template <class V, class U>
+__device__ __forceinline__ V unset_like(U u) {
+  V v;
+  asm("; unset %0 (%1)" : "=v"(v) : "v"(u));
+  return v;
+}
+
 // Two FP32 values (even and odd cell of one pair) stochastically rounded to
-// packed 16-bit storage with the pair's random word h.
+// packed 16-bit storage with the pair's random word h.  Each conversion writes
+// one half of the word and keeps the other, so the word starts unset (no
+// initialising move; the register can be the store's own).
 template <int S>
 __device__ __forceinline__ uint32_t sr_pack_pair(float lo, float hi, uint32_t h) {
   if constexpr (S == kBF16) {
-    bf16x2 v = {};
+    bf16x2 v = unset_like<bf16x2>(lo);
     v = __builtin_amdgcn_cvt_sr_bf16_f32(v, lo, h, false);
     v = __builtin_amdgcn_cvt_sr_bf16_f32(v, hi, h << 16, true);
     return __builtin_bit_cast(uint32_t, v);
   } else {
-    f16x2 v = {};
+    f16x2 v = unset_like<f16x2>(lo);
     v = __builtin_amdgcn_cvt_sr_f16_f32(v, lo, h, false);
     v = __builtin_amdgcn_cvt_sr_f16_f32(v, hi, h << 16, true);
     return __builtin_bit_cast(uint32_t, v);
@@ -382,7 +395,7 @@ struct TileRunner {
   int ldoff[RPW][G];   // byte offset in a plane, or kOutOfRange (loads zeros)
   int stoff[RPW][G];   // byte offset in a plane, or kOutOfRange (store dropped)
   bool rowlast[RPW];
-  bool rowhalo[RPW];    // row is not stored (uniform per wave)
+  int rowhalo[RPW];     // row is not stored: 0 / 1, uniform per wave (a scalar register)
   uint32_t sr_lane[G];  // 16-bit storage: Weyl offset of the lane's first x pair per group
 
   __device__ __forceinline__ TileRunner(const ST *s_, ST *d_, const Geometry &g, const Coef<C> &k_,
@@ -413,7 +426,7 @@ struct TileRunner {
       const bool rowin = y < geo.ny;
       const bool rowst = y >= yo0 && y < yo1;
       rowlast[r] = y == geo.ny - 1;
-      rowhalo[r] = __builtin_amdgcn_readfirstlane(rowst ? 0 : 1) != 0;
+      rowhalo[r] = __builtin_amdgcn_readfirstlane(rowst ? 0 : 1);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int xv = xlane + g * GSTRIDE;
@@ -503,14 +516,26 @@ struct TileRunner {
     constexpr int sc = modn<NS>(PH - s), sb = modn<NS>(PH - s - 1), st = modn<NS>(PH - s + 1);
     const bool zb = (p == 0), zt = (p == geo.nz - 1);
     const int wn = max(wave - 1, 0), ws = min(wave + 1, NW - 1);
+    constexpr bool k16 = (S == kBF16 || S == kF16);
     C out[RPW][NV];
+    // 16-bit storage: every row is rounded to packed storage words (nearest
+    // even or stochastic; the branch on geo.sr only touches registers), which
+    // are then either unpacked as the next level or stored.  One store site,
+    // so the memory instructions of an iteration do not depend on the mode.
+    uint32_t wv[RPW][k16 ? NV / 2 : 1];
+    const uint32_t sbase = (uint32_t)(geo.step0 + s - 1) * kSrStepMul + geo.seed;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const C *cv = lv[s - 1][sc][r];
-      if (s == T && kSkipHaloRows && rowhalo[r]) {
+      if (s == T && kSkipHaloRows && rowhalo[r] != 0) {
         // uniform branch: the store of this row is dropped anyway
+        if constexpr (k16) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) out[r][v] = cv[v];
+          for (int w = 0; w < NV / 2; ++w) wv[r][w] = unset_like<uint32_t>(cv[2 * w]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) out[r][v] = unset_like<C>(cv[v]);
+        }
         continue;
       }
       // rows above / below this wave's first / last row
@@ -593,22 +618,8 @@ struct TileRunner {
           }
         }
       }
-    }
-    if constexpr (S == kBF16 || S == kF16) {
-      // 16-bit storage: round every row to packed storage words (nearest even
-      // or stochastic; the branch on geo.sr only touches registers), then
-      // either unpack them as the next level or store them.  One store site,
-      // so the memory instructions of an iteration do not depend on the mode.
-      uint32_t wv[RPW][NV / 2];
-      if (geo.sr) {
-        const uint32_t sbase = (uint32_t)(geo.step0 + s - 1) * kSrStepMul + geo.seed;
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-          if (s == T && kSkipHaloRows && rowhalo[r]) {
-#pragma unroll
-            for (int w = 0; w < NV / 2; ++w) wv[r][w] = 0u;  // the store is dropped
-            continue;
-          }
+      if constexpr (k16) {
+        if (geo.sr) {
           // uniform per wave: the scalar unit computes it
           const uint32_t row = (uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r);
           const uint32_t base = lowbias32(row * kSrRowMul + sbase);
@@ -620,13 +631,13 @@ struct TileRunner {
               wv[r][(g * VEC + v) / 2] = sr_pack_pair<S>(out[r][g * VEC + v], out[r][g * VEC + v + 1],
                                                           hb + (uint32_t)(v / 2) * kSrPairWeyl);
           }
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
+        } else {
 #pragma unroll
           for (int g = 0; g < G; ++g) P::pack(&out[r][g * VEC], &wv[r][g * VEC / 2]);
+        }
       }
+    }
+    if constexpr (k16) {
       if constexpr (s < T) {
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
@@ -692,10 +703,12 @@ struct TileRunner {
 
   __device__ __forceinline__ void run() {
     // the first plane (zl0) is requested at iteration zl0 - PF (PF = 0: at
-    // iteration zl0 - 1, after its first stage); start at the multiple of NS
-    // at or below it so phases are compile-time
-    const int a = zl0 - (PF > 0 ? PF : 1);
-    const int i0 = a - modn<NS>(a);
+    // iteration zl0 - 1, after its first stage).  Phases (register slots)
+    // count from that iteration; the loop runs whole groups of NS iterations,
+    // so a chunk whose iteration count is not a multiple of NS computes up to
+    // NS - 1 planes past its end (their stores are dropped).  The host sizes
+    // interior chunks to whole groups.
+    const int i0 = zl0 - (PF > 0 ? PF : 1);
     const int ilast = zo1 - 1 + T;
     for (int i = i0; i <= ilast; i += NS) iters<0>(i);
   }
