@@ -113,11 +113,13 @@ int make_geometry(const KernelEntry &e, int nx, int ny, int nz, int pitch, int z
   q.oz = (nz + nzc - 1) / nzc;
   if (nzc > 1) {
     // A chunk runs its z loop in whole groups of NS iterations (NS = the
-    // register-slot count, PF + 3, or 3 for the ring), oz + 2T + max(PF, 1)
-    // of them for an interior chunk: round oz up so that is a multiple of NS
-    // and no interior chunk computes planes past its end.
+    // register-slot count, PF + 3, or 3 for the ring), oz + T + H + max(PF, 1)
+    // of them for an interior chunk with H halo planes (T, or ceil(T / RPW)
+    // for the 2D kernels' row blocks): round oz up so that is a multiple of
+    // NS and no interior chunk computes planes past its end.
     const int ns = e.pf > 0 ? e.pf + 3 : 3;
-    const int lead = 2 * T + std::max(e.pf, 1);
+    const int halo = e.rows ? (T + e.RPW - 1) / e.RPW : T;
+    const int lead = T + halo + std::max(e.pf, 1);
     while ((q.oz + lead) % ns != 0) ++q.oz;
   }
   q.nzc = (nz + q.oz - 1) / q.oz;
@@ -172,7 +174,7 @@ int field_geometry(const KernelEntry &e, const d3h_field *f, int zchunks, Geomet
   const int planes = (f->ny + e.RPW - 1) / e.RPW;
   if (zchunks <= 0) {
     // one full wave of resident workgroups, chunks of at least max(16T, 64)
-    // rows (a row is cheap, so the T halo blocks must stay a small share)
+    // rows (a row is cheap, so the halo blocks must stay a small share)
     D3H_TRY(make_geometry(e, f->nx, e.RPW, planes, f->pitch, 1, cus, occ, g, blocks));
     const long long slots = (long long)std::max(occ, 1) * std::max(cus, 1);
     const long long min_planes = (std::max(16 * e.T, 64) + e.RPW - 1) / e.RPW;

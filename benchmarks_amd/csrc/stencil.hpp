@@ -367,6 +367,9 @@ struct TileRunner {
   static constexpr int TILE_X = G * GSTRIDE;
   static constexpr int TILE_Y = NW * RPW;
   static constexpr int NS = PF + 3;
+  // Halo planes of a z-chunk: T (one plane per fused step), or in the 2D mode,
+  // where a step reaches one row and a "plane" holds RPW rows, ceil(T / RPW).
+  static constexpr int kZHalo = W2D ? (T + RPW - 1) / RPW : T;
   static_assert(PF > 0 || (T >= 2 && !W2D), "the three-slot ring needs fused depths");
   // A tile taller than its 2T halo rows has halo rows whose last-stage values
   // are never stored; the last stage skips their arithmetic.  (Skipping, at
@@ -412,8 +415,8 @@ struct TileRunner {
     const int yl0 = (geo.nty == 1) ? 0 : max(yo0 - T, 0);
     zo0 = tz * geo.oz;
     zo1 = min(zo0 + geo.oz, geo.nz);
-    zl0 = max(zo0 - T, 0);
-    zl1 = min(zo1 + T, geo.nz);
+    zl0 = max(zo0 - kZHalo, 0);
+    zl1 = min(zo1 + kZHalo, geo.nz);
     edge_xy = (xl0 + TILE_X > geo.nx) || (yl0 + TILE_Y > geo.ny);
     xlane = xl0 + lane * VEC;
     yrow0 = yl0 + wave * RPW;
@@ -489,10 +492,6 @@ struct TileRunner {
 #pragma unroll
       for (int g = 0; g < G; ++g) P::store_words(rs, stoff[r][g], &w[r][g * VEC / 2]);
   }
-
-  // First / last valid plane of level s in this chunk.
-  __device__ __forceinline__ int lo(int s) const { return zl0 > 0 ? zl0 + s : 0; }
-  __device__ __forceinline__ int hi(int s) const { return zl1 < geo.nz ? zl1 - 1 - s : geo.nz - 1; }
 
   // Publish the first/last row of every level's centre plane for iteration
   // phase PH (plane i - s of level s - 1 for stage s).
