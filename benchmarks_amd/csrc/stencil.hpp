@@ -86,7 +86,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, in
 //   Tiles overlap: tile (tx,ty,tz) stores x in [tx*ox, tx*ox+ox), rows
 //   [ty*oy, ty*oy+oy), planes [tz*oz, tz*oz+oz) (clipped to the grid) and loads
 //   a window that extends hx cells / T rows / T planes beyond that on every
-//   side that is not a domain boundary.  pitch is the padded row length
+//   side that is not a domain boundary (the last tile's window is moved back
+//   to end at the grid's edge where it fits, see TileRunner).  pitch is the padded row length
 //   (multiple of 16 bytes); plane = pitch * ny.
 struct Geometry {
   int nx, ny, nz, pitch;
@@ -416,10 +417,19 @@ struct TileRunner {
     const int xo0 = tx * geo.ox;
     const int xo1 = min(xo0 + geo.ox, geo.nx);
     const int xo1s = (xo1 == geo.nx) ? geo.pitch : xo1;  // row padding may be overwritten
-    const int xl0 = (geo.ntx == 1) ? 0 : max(xo0 - geo.hx, 0);
+    int xl0 = (geo.ntx == 1) ? 0 : max(xo0 - geo.hx, 0);
     const int yo0 = ty * geo.oy;
     const int yo1 = min(yo0 + geo.oy, geo.ny);
-    const int yl0 = (geo.nty == 1) ? 0 : max(yo0 - T, 0);
+    int yl0 = (geo.nty == 1) ? 0 : max(yo0 - T, 0);
+    // A window that would reach past the grid's last column (row) is moved
+    // back to end exactly there, when the grid is at least a tile wide (tall)
+    // and the shifted start stays on a lane-vector boundary: its stored cells
+    // still lie T or more cells inside the window, and the last lane's (last
+    // wave's) own cell is then the clamped neighbour, so the tile needs no
+    // clamp selects.
+    if (xl0 + TILE_X > geo.nx && geo.nx >= TILE_X && (geo.nx - TILE_X) % VEC == 0)
+      xl0 = geo.nx - TILE_X;
+    if (yl0 + TILE_Y > geo.ny && geo.ny >= TILE_Y) yl0 = geo.ny - TILE_Y;
     zo0 = tz * geo.oz;
     zo1 = min(zo0 + geo.oz, geo.nz);
     zl0 = max(zo0 - kZHalo, 0);

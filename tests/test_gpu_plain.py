@@ -81,7 +81,7 @@ def test_config1_128_cubed(ctx, golden):
         assert "%e" % oracle.accuracy_f32(out, k, 100) == case["accuracy"] == "7.227031e-07"
 
 
-@pytest.mark.parametrize("shape", [(300, 37, 23), (520, 70, 41), (64, 3, 2), (1, 1, 1), (7, 1, 9)])
+@pytest.mark.parametrize("shape", [(300, 37, 23), (301, 37, 23), (520, 70, 41), (64, 3, 2), (1, 1, 1), (7, 1, 9)])
 def test_every_plain_tile_exact_matches_oracle(ctx, shape):
     k = d3.constants_f32(*shape)
     f0 = d3.initialize_f32(*shape, k)
