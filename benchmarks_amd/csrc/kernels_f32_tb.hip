@@ -20,9 +20,7 @@ namespace d3h {
   /* one cell per lane: 64-wide tiles for narrow (batched) grids */              \
   D3H_ENTRY_B(kF32, 1, 2, 1, 4, 8, A), D3H_ENTRY_B(kF32, 1, 4, 1, 4, 8, A),       \
   /* three-slot register ring (prefetch 0): taller tiles at the same occupancy */ \
-  D3H_ENTRY_PF(kF32, 2, 4, 1, 4, 8, A, 0), D3H_ENTRY_PF(kF32, 2, 8, 1, 4, 8, A, 0),     \
-  /* 64-row depth-4 ring tile: 16 waves, 56 of 64 rows stored */                  \
-  D3H_ENTRY_PF(kF32, 2, 4, 1, 4, 16, A, 0)
+  D3H_ENTRY_PF(kF32, 2, 4, 1, 4, 8, A, 0), D3H_ENTRY_PF(kF32, 2, 8, 1, 4, 8, A, 0)
 
 static const KernelEntry kTable[] = {F32_TB(false), F32_TB(true)};
 

@@ -687,7 +687,10 @@ struct TileRunner {
 
   // Every stage runs every iteration (planes outside the valid range compute
   // values that are never stored or read by a valid plane), so each path
-  // through an iteration issues the same memory instructions.
+  // through an iteration issues the same memory instructions.  (Skipping the
+  // arithmetic of the planes no stored plane needs, behind a uniform branch
+  // that still issued the dropped stores, measured 7-10 % slower on the 512^3
+  // BF16 and 4-wave FP32 tiles and changed depths 2 and 4 by under 1 %.)
   template <int PH, int s>
   __device__ __forceinline__ void stage(int i, int buf) {
     const int p = i - s;
