@@ -28,6 +28,14 @@ closed-loop solves/s, p50/p99 latency at --served-rate arrivals/s, solves/s
 of the plain kernel launched per request, whether every result equals the
 per-request result bitwise, and FP32 / BF16 errors against the FP64 GPU run.
 
+temporal_blocking and batched (default workload only, --extras 0 skips them):
+bounded runs of the other two benchmark configurations under the same clock --
+the 768^3 FP32 field with seeded noise at depth 4 against plain steps
+(--tb-steps steps each, effective Gcell/s, bitwise equality), and the batch of
+512 uneven grids (200 steps) in FP32 and BF16 storage (solves/s end to end and
+device-resident, against the plain kernel per request, bitwise equality,
+errors against the FP64 GPU run).
+
 error_vs_fp64_gpu_oracle: relative L2 and max-abs difference of the FP32
 result from the same initial field and coefficients advanced in FP64 on the
 GPU (EXACT arithmetic); error_vs_analytical_fp64: RMS / max-abs against the
@@ -349,6 +357,63 @@ def served_summary(d3, ctx, args):
     }
 
 
+def tb_summary(d3, ctx, steps):
+    """Bounded temporal-blocking figures for the default line: the 768^3 FP32
+    field with seeded noise, depth 4 (autotuned tile), `steps` steps from the
+    same resident input as `steps` plain steps, whose result it must equal
+    bitwise."""
+    n = 768
+    k = d3.constants_f32(n, n, n)
+    f0 = d3.add_noise_f32(d3.initialize_f32(n, n, n, k), NOISE_SEED, 1e-3)
+    coef = d3.coefficients(k)
+    out = {"workload": "768^3 FP32 + seeded noise 1e-3, %d steps, depth 4 vs plain, exact "
+                       "arithmetic" % steps}
+    with d3.Field(ctx, n, n, n, d3.F32) as pristine, d3.Field(ctx, n, n, n, d3.F32) as f:
+        pristine.upload(f0)
+        del f0
+        ctx.autotune(n, n, n, d3.F32, 4, d3.ARITH_EXACT)
+        ctx.autotune(n, n, n, d3.F32, 1, d3.ARITH_EXACT)
+        res = {}
+        for depth in (4, 1):
+            f.copy_from(pristine)
+            f.run(8, coef, depth=depth)                 # warm-up
+            f.copy_from(pristine)
+            t0 = time.perf_counter()
+            tm = f.run(steps, coef, depth=depth)
+            wall = time.perf_counter() - t0
+            res[depth] = f.download()
+            cells = n ** 3
+            out["depth%d" % depth] = {
+                "gcell_updates_per_s": round(cells * steps / wall / 1e9, 2),
+                "avg_launch_ms": round(tm["avg_launch_ms"], 4),
+                "effective_frac_of_hbm_peak": round(
+                    8.0 * cells * tm["temporal_depth"] / (tm["avg_launch_ms"] * 1e-3) / 1e9
+                    / HBM_PEAK_GBPS, 4),
+                "tile": tm["tile"], "zchunks": tm["zchunks"], "launches": tm["launches"]}
+    out["speedup_depth4_vs_plain"] = round(out["depth4"]["gcell_updates_per_s"]
+                                           / out["depth1"]["gcell_updates_per_s"], 3)
+    out["bitwise_equal_to_plain"] = bool(np.array_equal(res[4].view(np.uint32),
+                                                        res[1].view(np.uint32)))
+    return out
+
+
+def batched_summary(args, d3, ctx):
+    """Bounded batched figures for the default line: the 512-grid batch (200
+    steps) in FP32 and in BF16 storage with stochastic rounding."""
+    out = {}
+    for dname in ("f32", "bf16"):
+        r = bench_batched(args, d3, ctx, dname)
+        out[dname] = {k: r[k] for k in (
+            "value", "device_resident_solves_per_s", "device_resident_gcell_updates_per_s",
+            "per_request_plain_solves_per_s", "speedup_vs_per_request",
+            "results_equal_per_request", "error_vs_fp64")}
+        out[dname]["solves_per_s"] = out[dname].pop("value")
+    out["workload"] = ("512 uneven grids (dims log-uniform [32,256], seeded), 200 steps, depth per "
+                       "size class; solves/s end to end (upload, run, download) vs the plain "
+                       "kernel launched per request")
+    return out
+
+
 def bench_served(args, d3, ctx):
     arith = d3.ARITH_FMA if args.arith == "fma" else d3.ARITH_EXACT
     depth = args.depth if args.depth is not None else 0
@@ -407,6 +472,11 @@ def main():
     ap.add_argument("--served-rate", type=float, default=50.0,
                     help="default workload: Poisson arrival rate (1/s) of the served sample's "
                          "open-loop run")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="default workload: add the bounded depth-4 768^3 and batched 512-grid "
+                         "objects to the line (0 = skip)")
+    ap.add_argument("--tb-steps", type=int, default=200,
+                    help="default workload: steps of the depth-4 768^3 comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU (this build has no multi-GPU path); "
@@ -541,8 +611,12 @@ def main():
                 t["dtype"], vec, t["temporal_depth"], t["groups"], t["rows_per_wave"], t["waves"],
                 fma, t["prefetch"])
         grid = tm["blocks"] * 64 * t["waves"]
-        if prof.get("kernel") == name and prof.get("grid_size", grid) == grid:
-            traffic = prof.get("hbm_bytes_per_launch")
+        # one profiled kernel, or a table of every kernel and grid the
+        # autotuner launched in the profiled runs ("dispatches")
+        for d in prof.get("dispatches", [prof]):
+            if d.get("kernel") == name and d.get("grid_size", grid) == grid:
+                traffic = d.get("hbm_bytes_per_launch")
+                break
 
     depth_used = tm["temporal_depth"]
     cpu_plain = None
@@ -550,6 +624,12 @@ def main():
     if args.workload == "plain512_f32" and args.served_requests > 0:
         served = served_summary(d3, ctx, args)
         print("served:", served, file=sys.stderr, flush=True)
+    tb_line = batched_line = None
+    if args.workload == "plain512_f32" and args.extras:
+        tb_line = tb_summary(d3, ctx, args.tb_steps)
+        print("temporal blocking:", tb_line, file=sys.stderr, flush=True)
+        batched_line = batched_summary(args, d3, ctx)
+        print("batched:", batched_line, file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -613,6 +693,8 @@ def main():
         "error_vs_fp64_gpu_oracle": ({"rel_l2": err64["rel_l2"], "max_abs": err64["max_abs"]}
                                      if err64 else None),
         "served": served,
+        "temporal_blocking": tb_line,
+        "batched": batched_line,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
