@@ -281,6 +281,32 @@ __device__ __forceinline__ double dpp(double old, double src) {
   int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(src), CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
+// coef * (the neighbour lane's src) in one instruction: v_mul_f32 with a DPP
+// wave_shr:1 / wave_shl:1 source.  The lane without a source keeps `old` (the
+// instruction's destination is tied to it and no bound_ctrl is set), so the
+// caller passes old = coef * (its own clamp value) and gets the product the
+// separate move, DPP move and multiply gave, bitwise (one rounding of the
+// same product).  coef_v is a VGPR (VOP2 DPP takes no scalar second operand).
+// The s_nop gives the 2 wait states a DPP read of a VGPR needs after a VALU
+// write of it, since the compiler does not look inside the statement.
+template <int CTRL>
+__device__ __forceinline__ float mul_dpp(float old, float src, float coef_v) {
+  float d = old;
+  if constexpr (CTRL == kWaveShr1)
+    asm("s_nop 1\n\tv_mul_f32_dpp %0, %1, %2 wave_shr:1 row_mask:0xf bank_mask:0xf"
+        : "+v"(d) : "v"(src), "v"(coef_v));
+  else
+    asm("s_nop 1\n\tv_mul_f32_dpp %0, %1, %2 wave_shl:1 row_mask:0xf bank_mask:0xf"
+        : "+v"(d) : "v"(src), "v"(coef_v));
+  return d;
+}
+// A copy of a uniform value that stays in a VGPR (the empty statement hides
+// that it is uniform, so it is not rematerialised from the scalar per use).
+__device__ __forceinline__ float in_vgpr(float v) {
+  float r;
+  asm("; vgpr copy %0" : "=v"(r) : "0"(v));
+  return r;
+}
 __device__ __forceinline__ float readlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -386,6 +412,12 @@ struct TileRunner {
   // 1 % (FP32, depth 2) and 5 % (BF16) faster, 768^3 depth 2 2 % faster.  The
   // deep tiles keep the branch: depth 4 measured 5 % slower with the reads.
   static constexpr bool kEdgeRowsFromLds = T <= 2;
+  // EXACT FP32 arithmetic (also for 16-bit storage) multiplies the x terms of
+  // a lane's edge cells in the DPP instruction that fetches the neighbour
+  // (mul_dpp): 2 VALU instructions per row and stage fewer than a move of the
+  // clamp value, a DPP move and a multiply for each side.  The EDGE variant
+  // keeps the moves (its clamp selects need the neighbour's value).
+  static constexpr bool kXMul = !FMA && S != kF64;
 
   // lv[level][slot][row][cell]: level 0 is the loaded input.
   C lv[T][NS][RPW][NV];
@@ -408,6 +440,7 @@ struct TileRunner {
   bool rowlast[RPW];
   int rowhalo[RPW];     // row is not stored: 0 / 1, uniform per wave (a scalar register)
   uint32_t sr_lane[G];  // 16-bit storage: Weyl offset of the lane's first x pair per group
+  float cwv, cev;       // cw and ce in VGPRs (mul_dpp's second operand)
 
   __device__ __forceinline__ TileRunner(const ST *s_, ST *d_, const Geometry &g, const Coef<C> &k_,
                                         C *x_, int tx, int ty, int tz)
@@ -454,6 +487,10 @@ struct TileRunner {
         ldoff[r][g] = (rowin && xv < geo.pitch) ? off : kOutOfRange;
         stoff[r][g] = (rowst && xv >= xo0 && xv < xo1s) ? off : kOutOfRange;
       }
+    }
+    if constexpr (kXMul) {
+      cwv = in_vgpr((float)k.cw);
+      cev = in_vgpr((float)k.ce);
     }
     if constexpr (S == kBF16 || S == kF16) {
       // xlane is a multiple of VEC (even), so a lane's x pairs are its own
@@ -615,6 +652,35 @@ struct TileRunner {
         // x neighbours across lanes (group seams through readlane)
         const C wold = (g > 0) ? readlane(cv[(g - 1) * VEC + VEC - 1], 63) : cv[0];
         const C eold = (g < G - 1) ? readlane(cv[(g + 1) * VEC], 0) : cv[g * VEC + VEC - 1];
+        if constexpr (kXMul && !EDGE) {
+          // The reference's sum, cc c + cw w + ce e + ..., accumulated term by
+          // term; the x terms of the lane's first and last cells are multiplied
+          // as they cross lanes (the lane without a neighbour keeps coef * its
+          // old value).  The in-lane terms come first: for group 0 the DPP's
+          // old values cw * c0 and ce * c_last are also cell 1's w term and
+          // cell VEC-2's e term, and are then dead when the DPP overwrites
+          // them (no copies).
+          const int g0 = g * VEC;
+          C acc[VEC];
+#pragma unroll
+          for (int v = 1; v < VEC; ++v) acc[v] = k.cc * cv[g0 + v] + k.cw * cv[g0 + v - 1];
+          const C pw = mul_dpp<kWaveShr1>(k.cw * wold, cv[g0 + VEC - 1], cwv);
+          acc[0] = k.cc * cv[g0] + pw;
+#pragma unroll
+          for (int v = 0; v < VEC - 1; ++v) acc[v] = acc[v] + k.ce * cv[g0 + v + 1];
+          const C pe = mul_dpp<kWaveShl1>(k.ce * eold, cv[g0], cev);
+          acc[VEC - 1] = acc[VEC - 1] + pe;
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            const int gv = g0 + v;
+            if constexpr (W2D)
+              out[r][gv] = acc[v] + k.cs * srow[gv] + k.cn * nrow[gv];
+            else
+              out[r][gv] = acc[v] + k.cs * srow[gv] + k.cn * nrow[gv] + k.cb * lv[s - 1][sb][r][gv] +
+                           k.ct * lv[s - 1][st][r][gv];
+          }
+          continue;
+        }
         const C wl = dpp<kWaveShr1>(wold, cv[g * VEC + VEC - 1]);
         const C er = dpp<kWaveShl1>(eold, cv[g * VEC]);
 #pragma unroll
