@@ -382,9 +382,18 @@ struct XchLayout {
 // r +- 1 of the same plane, or, across a plane boundary, the last row of the
 // previous plane / the first row of the next one (register slots of the
 // adjacent planes); there are no z terms.
-template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF, bool W2D = false>
+//
+// REV (2D mode only): the chunk streams its row blocks from the last to the
+// first.  Stream index q is then row block nz - 1 - q (the chunk's ranges are
+// mirrored), and a block's neighbour blocks swap register slots.  Odd chunks
+// stream backwards, so the boundary blocks two adjacent chunks both read are
+// read by both at about the same time (at the end of both, or at the start of
+// both) and the second read is served from L2.
+template <int S, int VEC, int T, int G, int RPW, int NW, bool FMA, int PF, bool W2D = false,
+          bool REV = false>
 struct TileRunner {
   static_assert(!W2D || NW == 1, "2D row blocks are streamed by one wave");
+  static_assert(!REV || W2D, "only 2D row blocks stream backwards");
   using ST = typename Traits<S>::T;
   using C = typename Traits<S>::C;
   static constexpr int NV = G * VEC;
@@ -465,6 +474,11 @@ struct TileRunner {
     if (yl0 + TILE_Y > geo.ny && geo.ny >= TILE_Y) yl0 = geo.ny - TILE_Y;
     zo0 = tz * geo.oz;
     zo1 = min(zo0 + geo.oz, geo.nz);
+    if constexpr (REV) {
+      const int r0 = zo0;
+      zo0 = geo.nz - zo1;
+      zo1 = geo.nz - r0;
+    }
     zl0 = max(zo0 - kZHalo, 0);
     zl1 = min(zo1 + kZHalo, geo.nz);
     edge_xy = (xl0 + TILE_X > geo.nx) || (yl0 + TILE_Y > geo.ny);
@@ -507,6 +521,9 @@ struct TileRunner {
           for (int v = 0; v < NV; ++v) lv[l][s][r][v] = C(0);
   }
 
+  // The field's plane (2D: row block) at stream index q.
+  __device__ __forceinline__ int real(int q) const { return REV ? geo.nz - 1 - q : q; }
+
   // Bytes of plane q that belong to the field: the whole plane, except the
   // clipped last row block of a 2D field (rows beyond it read zeros and their
   // stores are dropped).
@@ -519,7 +536,8 @@ struct TileRunner {
   template <int SL>
   __device__ __forceinline__ void load_plane(int q) {
     const bool in = q >= zl0 && q < zl1;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(src + (long long)q * geo.plane, in ? valid_bytes(q) : 0);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(src + (long long)real(q) * geo.plane, in ? valid_bytes(real(q)) : 0);
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
 #pragma unroll
@@ -529,7 +547,8 @@ struct TileRunner {
   // Store plane q; planes outside [zo0, zo1) are dropped.
   __device__ __forceinline__ void store_plane(int q, C (&out)[RPW][NV]) {
     const bool in = q >= zo0 && q < zo1;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(dst + (long long)q * geo.plane, in ? valid_bytes(q) : 0);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(dst + (long long)real(q) * geo.plane, in ? valid_bytes(real(q)) : 0);
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
 #pragma unroll
@@ -540,7 +559,8 @@ struct TileRunner {
   // [zo0, zo1) are dropped.
   __device__ __forceinline__ void store_plane_words(int q, const uint32_t (&w)[RPW][NV / 2]) {
     const bool in = q >= zo0 && q < zo1;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(dst + (long long)q * geo.plane, in ? valid_bytes(q) : 0);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(dst + (long long)real(q) * geo.plane, in ? valid_bytes(real(q)) : 0);
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
 #pragma unroll
@@ -567,7 +587,8 @@ struct TileRunner {
   template <int PH, int s, bool EDGE>
   __device__ __forceinline__ void stage_impl(int p, int buf) {
     constexpr int sc = modn<NS>(PH - s), sb = modn<NS>(PH - s - 1), st = modn<NS>(PH - s + 1);
-    const bool zb = (p == 0), zt = (p == geo.nz - 1);
+    // the field's first / last plane (row block)
+    const bool zb = (real(p) == 0), zt = (real(p) == geo.nz - 1);
     const int wn = max(wave - 1, 0), ws = min(wave + 1, NW - 1);
     constexpr bool k16 = (S == kBF16 || S == kF16);
     C out[RPW][NV];
@@ -594,10 +615,12 @@ struct TileRunner {
       // rows above / below this wave's first / last row
       C nrow[NV], srow[NV];
       if (W2D && r == 0) {
-        // the previous plane's last row (its value at a field edge is
-        // replaced by the cell itself below)
+        // the previous block's last row (its value at a field edge is
+        // replaced by the cell itself below); streaming backwards, the block
+        // above is the next one in the stream
+        constexpr int sn = REV ? st : sb;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) nrow[v] = lv[s - 1][sb][RPW - 1][v];
+        for (int v = 0; v < NV; ++v) nrow[v] = lv[s - 1][sn][RPW - 1][v];
       } else if (r == 0) {
         if constexpr (NW > 1 && kEdgeRowsFromLds) {
           // the previous wave's last row; wave 0 reads back its own first row
@@ -622,9 +645,11 @@ struct TileRunner {
         for (int v = 0; v < NV; ++v) nrow[v] = lv[s - 1][sc][r - 1][v];
       }
       if (W2D && r == RPW - 1) {
-        // the next plane's first row
+        // the next block's first row (streaming backwards: the previous one
+        // in the stream)
+        constexpr int ss = REV ? sb : st;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) srow[v] = lv[s - 1][st][0][v];
+        for (int v = 0; v < NV; ++v) srow[v] = lv[s - 1][ss][0][v];
       } else if (r == RPW - 1) {
         if constexpr (NW > 1 && kEdgeRowsFromLds) {
           // the next wave's first row; the last wave reads back its own last row
@@ -714,7 +739,7 @@ struct TileRunner {
       if constexpr (k16) {
         if (geo.sr) {
           // uniform per wave: the scalar unit computes it
-          const uint32_t row = (uint32_t)p * (uint32_t)geo.ny + (uint32_t)(yrow0 + r);
+          const uint32_t row = (uint32_t)real(p) * (uint32_t)geo.ny + (uint32_t)(yrow0 + r);
           const uint32_t base = lowbias32(row * kSrRowMul + sbase);
 #pragma unroll
           for (int g = 0; g < G; ++g) {
@@ -840,12 +865,19 @@ __global__ void __launch_bounds__(64)
 stencil2d_kernel(const typename Traits<S>::T *__restrict__ src, typename Traits<S>::T *__restrict__ dst,
                  Geometry geo, Coef<typename Traits<S>::C> k) {
   using R = TileRunner<S, VEC, T, G, RPW, 1, FMA, PF, true>;
+  using RR = TileRunner<S, VEC, T, G, RPW, 1, FMA, PF, true, true>;
   __shared__ typename Traits<S>::C xch[R::X::kElems];
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   const int tx = lb % geo.ntx;
   const int tz = lb / geo.ntx;  // one row-block tile per x tile (nty = 1)
-  R tr(src, dst, geo, k, xch, tx, 0, tz);
-  tr.run();
+  // odd chunks stream backwards (see TileRunner)
+  if (tz & 1) {
+    RR tr(src, dst, geo, k, xch, tx, 0, tz);
+    tr.run();
+  } else {
+    R tr(src, dst, geo, k, xch, tx, 0, tz);
+    tr.run();
+  }
 }
 
 // One member problem of a batched launch (device memory).  Members of one
