@@ -408,12 +408,17 @@ struct TileRunner {
   static constexpr int kZHalo = W2D ? (T + RPW - 1) / RPW : T;
   static_assert(PF > 0 || (T >= 2 && !W2D), "the three-slot ring needs fused depths");
   // A tile taller than its 2T halo rows has halo rows whose last-stage values
-  // are never stored; the last stage skips their arithmetic.  (Skipping, at
-  // stages 1..T-1, the rows those stages need not compute either was measured
-  // slower: its branches cost the deep tiles registers or scheduling freedom.
-  // FP64 tiles, whose 16 bytes per cell keep them far from the VALU limit,
-  // measured 1.4 % slower with the skip, so they compute every row.)
+  // are never stored; the last stage skips their arithmetic.  (FP64 tiles,
+  // whose 16 bytes per cell keep them far from the VALU limit, measured 1.4 %
+  // slower with the skip, so they compute every row.)
   static constexpr bool kSkipHaloRows = TILE_Y > 2 * T && S != kF64;
+  // Depths 2 and 4 also skip, at stages 1..T-1, the rows whose level no
+  // stored row reads: row y at stage s feeds the stored rows only within
+  // T - s rows of them (the tile's first and last waves hold such rows).
+  // Same-box A/B: 768^3 depth 4 -2 %, depth 2 -0.5..-1.7 %, the batched
+  // 512-grid FP32 sweep (depth 2) +5 % solves/s; depth 8 measured 16 % slower
+  // with it and computes every row.
+  static constexpr bool kSkipEarlyRows = kSkipHaloRows && T >= 2 && T <= 4 && !W2D;
   // The first / last wave of a workgroup takes the cell itself as its outer
   // y neighbour (the clamp at the tile's edge).  At T <= 2 it reads that row
   // back from the exchange buffer (where it published it), so the choice is an
@@ -448,6 +453,7 @@ struct TileRunner {
   int stoff[RPW][G];   // byte offset in a plane, or kOutOfRange (store dropped)
   bool rowlast[RPW];
   int rowhalo[RPW];     // row is not stored: 0 / 1, uniform per wave (a scalar register)
+  int rowskip;          // bit (s - 1) * RPW + r: stage s need not compute row r (uniform)
   uint32_t sr_lane[G];  // 16-bit storage: Weyl offset of the lane's first x pair per group
   float cwv, cev;       // cw and ce in VGPRs (mul_dpp's second operand)
 
@@ -494,6 +500,7 @@ struct TileRunner {
       const bool rowst = y >= yo0 && y < yo1;
       rowlast[r] = y == geo.ny - 1;
       rowhalo[r] = __builtin_amdgcn_readfirstlane(rowst ? 0 : 1);
+
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int xv = xlane + g * GSTRIDE;
@@ -505,6 +512,19 @@ struct TileRunner {
     if constexpr (kXMul) {
       cwv = in_vgpr((float)k.cw);
       cev = in_vgpr((float)k.ce);
+    }
+    if constexpr (kSkipEarlyRows) {
+      int m = 0;
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        const int y = yl0 + wave * RPW + r;
+#pragma unroll
+        for (int st = 1; st < T; ++st) {
+          const bool need = y >= yo0 - (T - st) && y < yo1 + (T - st) && y < geo.ny;
+          if (!need) m |= 1 << ((st - 1) * RPW + r);
+        }
+      }
+      rowskip = __builtin_amdgcn_readfirstlane(m);
     }
     if constexpr (S == kBF16 || S == kF16) {
       // xlane is a multiple of VEC (even), so a lane's x pairs are its own
@@ -601,8 +621,11 @@ struct TileRunner {
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const C *cv = lv[s - 1][sc][r];
-      if (s == T && kSkipHaloRows && rowhalo[r] != 0) {
-        // uniform branch: the store of this row is dropped anyway
+      const bool skip = (s == T) ? (kSkipHaloRows && rowhalo[r] != 0)
+                                 : (kSkipEarlyRows && ((rowskip >> ((s - 1) * RPW + r)) & 1) != 0);
+      if (skip) {
+        // uniform branch: the store of this row is dropped anyway (last
+        // stage), or no row that will be stored reads this row's level
         if constexpr (k16) {
 #pragma unroll
           for (int w = 0; w < NV / 2; ++w) wv[r][w] = unset_like<uint32_t>(cv[2 * w]);
