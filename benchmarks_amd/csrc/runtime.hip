@@ -1188,9 +1188,10 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
   const double coef[7] = {0.1, 0.1, 0.1, 0.1, rows ? 0.0 : 0.1, rows ? 0.0 : 0.1, rows ? 0.6 : 0.4};
   const auto &reg = registry();
   std::vector<int> depths;
-  // depth 8 is searched only for 2D fields: in 3D its tiles keep 8 of 24
-  // rows after the 2 x 8 halo rows, and it measured slower than plain steps
-  // (768^3: 526 against 743 Gcell/s); explicit depth-8 runs stay available
+  // depth 8 is searched only for 2D fields: in 3D the register file fits
+  // only small depth-8 tiles, whose 2 x 8 halo rows leave few stored rows
+  // (768^3: 981 Gcell/s effective against about 1790 at depth 4 and 743 for
+  // plain steps); explicit depth-8 runs stay available
   if (D == 0) depths = nz == 1 ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1, 2, 4};
   else depths = {D};
   double best_all = 1e300;
