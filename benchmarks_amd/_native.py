@@ -34,13 +34,14 @@ P = ctypes.POINTER
 
 class RunConfig(ctypes.Structure):
     _fields_ = [("temporal_depth", c_int), ("arith", c_int), ("tile", c_int),
-                ("zchunks", c_int), ("use_graph", c_int), ("rounding", c_int), ("seed", c_uint)]
+                ("zchunks", c_int), ("use_graph", c_int), ("rounding", c_int), ("seed", c_uint),
+                ("pace_ns", c_int)]
 
 
 class Timing(ctypes.Structure):
     _fields_ = [("kernel_ms", c_double), ("wall_ms", c_double), ("avg_launch_ms", c_double),
                 ("launches", c_int), ("steps", c_int), ("temporal_depth", c_int),
-                ("tile", c_int), ("zchunks", c_int), ("blocks", c_int)]
+                ("tile", c_int), ("zchunks", c_int), ("blocks", c_int), ("pace_ns", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -196,8 +196,16 @@ int tuned_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, in
   return it != ctx->tuned.end() ? it->second.T : dflt;
 }
 
+int pace_ticks(const d3h_context *ctx, const KernelEntry &e, int pace_ns) {
+  if (pace_ns <= 0 || e.T != 1 || ctx->rtc_khz <= 0) return 0;
+  const long long t = (long long)pace_ns * ctx->rtc_khz * 256 / 1000000;
+  return (int)std::min<long long>(t, INT_MAX);
+}
+
 int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
-                  bool rows, const KernelEntry **entry, int *zchunks) {
+                  bool rows, const KernelEntry **entry, int *zchunks, int *pace_ns) {
+  // an explicit schedule (> 0) or none (< 0) overrides the tuned one
+  *pace_ns = cfg.pace_ns > 0 ? cfg.pace_ns : 0;
   int T = cfg.temporal_depth;
   const int fma = cfg.arith == D3H_ARITH_FMA ? 1 : 0;
   if (cfg.arith != D3H_ARITH_EXACT && cfg.arith != D3H_ARITH_FMA)
@@ -229,6 +237,7 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
         if (e && (!e->rows || rows)) {
           *entry = e;
           *zchunks = cfg.zchunks > 0 ? cfg.zchunks : v.zchunks;
+          if (cfg.pace_ns == 0 && cfg.zchunks <= 0) *pace_ns = v.pace_ns;
           return D3H_OK;
         }
       }
@@ -240,6 +249,7 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
       if (e && e->T == T && (!e->rows || rows)) {
         *entry = e;
         *zchunks = cfg.zchunks > 0 ? cfg.zchunks : it->second.zchunks;
+        if (cfg.pace_ns == 0 && cfg.zchunks <= 0) *pace_ns = it->second.pace_ns;
         return D3H_OK;
       }
     }
@@ -253,7 +263,8 @@ int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h
 }
 
 int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEntry &e, int zchunks,
-                  int sr, unsigned seed, hipStream_t stream, int *launches, unsigned *blocks_out) {
+                  int pace_ns, int sr, unsigned seed, hipStream_t stream, int *launches,
+                  unsigned *blocks_out) {
   const int T = e.T;
   const int nfull = count / T, rem = count % T;
   Geometry g{}, g1{};
@@ -269,6 +280,7 @@ int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEnt
   }
   g.sr = g1.sr = sr;
   g.seed = g1.seed = seed;
+  g.pace = pace_ticks(f->ctx, e, pace_ns);  // the plain tail runs unscheduled
   // step numbers matter only under stochastic rounding; keeping them 0
   // otherwise leaves captured graphs reusable across calls
   const long long base = sr ? f->steps_done : 0;
@@ -583,6 +595,7 @@ void d3h_run_config_default(d3h_run_config *cfg) {
   cfg->use_graph = 0;
   cfg->rounding = D3H_ROUND_NEAREST;
   cfg->seed = 0;
+  cfg->pace_ns = 0;
 }
 
 int d3h_context_create(int device, d3h_context **out) {
@@ -602,6 +615,9 @@ int d3h_context_create(int device, d3h_context **out) {
   }
   c->num_cus = prop.multiProcessorCount;
   c->lds_per_block = (int)prop.sharedMemPerBlock;
+  if (hipDeviceGetAttribute(&c->rtc_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+      c->rtc_khz < 0)
+    c->rtc_khz = 0;
   {
     const int rc = init_noalloc_scratch(device);
     if (rc != D3H_OK) {
@@ -893,13 +909,16 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
                 f->dtype);
   if (sr && f->steps_done + count > INT_MAX)
     return fail(D3H_ERR_INVALID, "step numbers of stochastic rounding exceed 2^31");
-  D3H_TRY(resolve_entry(ctx, f->nx, f->ny, f->nz, f->dtype, cfg, rows_ok(f, coef), &e, &zchunks));
+  int pace_ns = 0;
+  D3H_TRY(resolve_entry(ctx, f->nx, f->ny, f->nz, f->dtype, cfg, rows_ok(f, coef), &e, &zchunks,
+                        &pace_ns));
+  if (pace_ticks(ctx, *e, pace_ns) == 0) pace_ns = 0;
   int launches = 0;
   unsigned blocks = 0;
   float ms = 0.f;
   if (count > 0) {
     if (cfg.use_graph) {
-      std::vector<long long> key = {count, (long long)(intptr_t)e, zchunks, f->cur, sr,
+      std::vector<long long> key = {count, (long long)(intptr_t)e, zchunks, pace_ns, f->cur, sr,
                                     sr ? (long long)cfg.seed : 0, sr ? f->steps_done : 0};
       for (int i = 0; i < 7; ++i) {
         long long bits;
@@ -911,8 +930,8 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
         const int cur0 = f->cur;
         hipGraph_t graph = nullptr;
         D3H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_steps(f, count, coef, *e, zchunks, sr, cfg.seed, ctx->stream, &launches,
-                               &blocks);
+        int rc = enqueue_steps(f, count, coef, *e, zchunks, pace_ns, sr, cfg.seed, ctx->stream,
+                               &launches, &blocks);
         hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
         f->cur = cur0;
         if (rc != D3H_OK) {
@@ -935,7 +954,8 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
       if (launches & 1) f->cur ^= 1;
     } else {
       D3H_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-      D3H_TRY(enqueue_steps(f, count, coef, *e, zchunks, sr, cfg.seed, ctx->stream, &launches, &blocks));
+      D3H_TRY(enqueue_steps(f, count, coef, *e, zchunks, pace_ns, sr, cfg.seed, ctx->stream, &launches,
+                            &blocks));
       D3H_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     }
     D3H_HIP(hipEventSynchronize(ctx->ev1));
@@ -949,6 +969,7 @@ int d3h_field_run(d3h_field *f, int count, const double coef[7], const d3h_run_c
     timing->steps = count;
     timing->temporal_depth = e->T;
     timing->tile = (int)(e - registry().data());
+    timing->pace_ns = pace_ns;
     Geometry g{};
     unsigned nb = 0;
     if (field_geometry(*e, f, zchunks, &g, &nb) == D3H_OK) {
@@ -1082,9 +1103,10 @@ int d3h_tile_info(int index, d3h_tile_desc *out) {
 }
 
 // Time every tile of depth T (and its z-chunk candidates) on field f; the
-// fastest per step goes to *bi / *bz / *best (bi < 0 when none ran).
+// fastest per step goes to *bi / *bz / *best (bi < 0 when none ran), with its
+// plane schedule in *bp (depth 1; 0 = none).
 static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const double coef[7], int *bi,
-                      int *bz, double *best) {
+                      int *bz, double *best, int *bp) {
   const auto &reg = registry();
   const bool rows = rows_ok(f, coef);
   const long long cells = (long long)f->nx * f->ny * f->nz;
@@ -1102,6 +1124,7 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
     Geometry g;
     unsigned blocks;
     double best;
+    int pace_ns;
   };
   std::vector<Cand> cands;
   for (int i = 0; i < (int)reg.size(); ++i) {
@@ -1127,36 +1150,78 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
       if (field_geometry(e, f, zc, &g2, &nb2) != D3H_OK) continue;
       if (g2.nzc == last_nzc) continue;  // same launch as the previous count
       last_nzc = g2.nzc;
-      cands.push_back(Cand{i, g2, nb2, 1e300});
+      cands.push_back(Cand{i, g2, nb2, 1e300, 0});
     }
   }
   // 16-bit storage is timed with stochastic rounding, the mode this problem
   // needs (nearest even stalls its small per-step changes) and the costlier one
   if (f->dtype == D3H_BF16 || f->dtype == D3H_F16)
     for (Cand &c : cands) c.g.sr = 1;
-  for (int pass = 0; pass < kPasses; ++pass) {
-    for (Cand &c : cands) {
-      const KernelEntry &e = reg[c.entry];
-      launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], c.g, coef, c.blocks, ctx->stream);  // warm-up
-      f->cur ^= 1;
-      (void)hipEventRecord(ctx->ev0, ctx->stream);
-      for (int r = 0; r < reps; ++r) {
-        launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], c.g, coef, c.blocks, ctx->stream);
+  auto time_all = [&](std::vector<Cand> &cs) -> int {
+    for (int pass = 0; pass < kPasses; ++pass) {
+      for (Cand &c : cs) {
+        const KernelEntry &e = reg[c.entry];
+        launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], c.g, coef, c.blocks, ctx->stream);  // warm-up
         f->cur ^= 1;
+        (void)hipEventRecord(ctx->ev0, ctx->stream);
+        for (int r = 0; r < reps; ++r) {
+          launch_entry(e, f->buf[f->cur], f->buf[f->cur ^ 1], c.g, coef, c.blocks, ctx->stream);
+          f->cur ^= 1;
+        }
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        if (hipEventSynchronize(ctx->ev1) != hipSuccess || hipGetLastError() != hipSuccess)
+          return fail(D3H_ERR_HIP, "autotune launch of tile %d failed", c.entry);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        c.best = std::min(c.best, (double)ms / reps / T);
       }
-      (void)hipEventRecord(ctx->ev1, ctx->stream);
-      if (hipEventSynchronize(ctx->ev1) != hipSuccess || hipGetLastError() != hipSuccess)
-        return fail(D3H_ERR_HIP, "autotune launch of tile %d failed", c.entry);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-      c.best = std::min(c.best, (double)ms / reps / T);
     }
-  }
+    return D3H_OK;
+  };
+  D3H_TRY(time_all(cands));
+  *bp = 0;
   for (const Cand &c : cands) {
     if (c.best < *best) {
       *best = c.best;
       *bi = c.entry;
       *bz = c.g.nzc;
+    }
+  }
+  // Depth 1: plane schedules (Geometry::pace) for the three fastest
+  // configurations, at 0.86-0.99 of each one's measured time per z iteration.
+  // A schedule that only the fastest workgroups reach keeps the streams of all
+  // workgroups on nearby planes; its optimum is narrow (on the 512^3 FP32 step
+  // about 0.93 of the unscheduled time per iteration for the 4-wave tile: 4-5 %
+  // faster; 0.97 for the 1-wave tile: under 1 %), and a slower schedule only
+  // loses.  One is kept only when it beats the best unscheduled time by more
+  // than 0.5 %.
+  if (T == 1 && ctx->rtc_khz > 0 && !cands.empty()) {
+    std::vector<Cand> top = cands;
+    std::sort(top.begin(), top.end(), [](const Cand &a, const Cand &b) { return a.best < b.best; });
+    if (top.size() > 3) top.resize(3);
+    std::vector<Cand> paced;
+    for (const Cand &c : top) {
+      const KernelEntry &e = reg[c.entry];
+      const int halo = e.rows ? (T + e.RPW - 1) / e.RPW : T;
+      const int iters = c.g.oz + T + halo + std::max(e.pf, 1);
+      const double ns_iter = c.best * 1e6 / iters;
+      for (int pct = 86; pct <= 99; ++pct) {
+        Cand p = c;
+        p.best = 1e300;
+        p.pace_ns = (int)(ns_iter * pct / 100.0);
+        p.g.pace = pace_ticks(ctx, e, p.pace_ns);
+        if (p.g.pace > 0) paced.push_back(p);
+      }
+    }
+    D3H_TRY(time_all(paced));
+    const double unscheduled = *best;
+    for (const Cand &p : paced) {
+      if (p.best < unscheduled * 0.995 && p.best < *best) {
+        *best = p.best;
+        *bi = p.entry;
+        *bz = p.g.nzc;
+        *bp = p.pace_ns;
+      }
     }
   }
   return D3H_OK;
@@ -1195,24 +1260,25 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
   if (D == 0) depths = nz == 1 ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1, 2, 4};
   else depths = {D};
   double best_all = 1e300;
-  int bi_all = -1, bz_all = 0;
+  int bi_all = -1, bz_all = 0, bp_all = 0;
   int rc = D3H_OK;
   for (int T : depths) {
     double best = 1e300;
-    int bi = -1, bz = 0;
-    rc = tune_depth(ctx, f, T, fma, coef, &bi, &bz, &best);
+    int bi = -1, bz = 0, bp = 0;
+    rc = tune_depth(ctx, f, T, fma, coef, &bi, &bz, &best, &bp);
     if (rc != D3H_OK) break;
     if (bi < 0) continue;
     const KernelEntry &e = reg[bi];
     {
       std::lock_guard<std::mutex> lk(ctx->mu);
       ctx->tuned[tune_key(dtype, T, fma, nx, ny, nz)] =
-          TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz, best, T, e.rows};
+          TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz, best, T, e.rows, bp};
     }
     if (best < best_all) {
       best_all = best;
       bi_all = bi;
       bz_all = bz;
+      bp_all = bp;
     }
   }
   d3h_field_destroy(f);
@@ -1222,7 +1288,7 @@ int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype, int tempor
     const KernelEntry &e = reg[bi_all];
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->tuned[tune_key(dtype, 0, fma, nx, ny, nz)] =
-        TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz_all, best_all, e.T, e.rows};
+        TuneVal{e.vec, e.G, e.RPW, e.NW, e.pf, bz_all, best_all, e.T, e.rows, bp_all};
   }
   if (best_tile) *best_tile = bi_all;
   if (best_zchunks) *best_zchunks = bz_all;
@@ -1235,15 +1301,16 @@ int d3h_autotune_save(d3h_context *ctx, const char *path) {
   std::ofstream os(path);
   if (!os) return fail(D3H_ERR_INVALID, "cannot write %s", path);
   os << "# dtype depth fma log2nx log2ny log2nz vec groups rows_per_wave waves prefetch zchunks "
-        "ms_per_step entry_depth batch rows   (depth 0 = the depth chosen for the size class; "
-        "batch 1 = chosen for batched sweeps; rows 1 = a 2D entry)\n";
+        "ms_per_step entry_depth batch rows pace_ns   (depth 0 = the depth chosen for the size "
+        "class; batch 1 = chosen for batched sweeps; rows 1 = a 2D entry; pace_ns = the plane "
+        "schedule of depth-1 launches, 0 = none)\n";
   std::lock_guard<std::mutex> lk(ctx->mu);
   for (const auto &kv : ctx->tuned) {
     const auto &k = kv.first;
     const auto &v = kv.second;
     os << k.dtype << ' ' << k.T << ' ' << k.fma << ' ' << k.cx << ' ' << k.cy << ' ' << k.cz << ' '
        << v.vec << ' ' << v.G << ' ' << v.RPW << ' ' << v.NW << ' ' << v.pf << ' ' << v.zchunks << ' '
-       << v.ms_per_step << ' ' << v.T << ' ' << k.batch << ' ' << v.rows << '\n';
+       << v.ms_per_step << ' ' << v.T << ' ' << k.batch << ' ' << v.rows << ' ' << v.pace_ns << '\n';
   }
   return os ? D3H_OK : fail(D3H_ERR_INVALID, "write to %s failed", path);
 }
@@ -1263,11 +1330,15 @@ int d3h_autotune_load(d3h_context *ctx, const char *path) {
           v.pf >> v.zchunks >> v.ms_per_step))
       return fail(D3H_ERR_INVALID, "malformed line in %s: %s", path, line.c_str());
     // entry depth (absent in tables without depth-0 lines: the key's depth),
-    // the batch flag (absent in tables without batched choices: 0), then the
-    // 2D flag (absent in tables written before 2D entries were told apart: 0)
+    // the batch flag (absent in tables without batched choices: 0), the 2D
+    // flag (absent in tables written before 2D entries were told apart: 0),
+    // then the plane schedule (absent in tables written before schedules: 0)
     if (!(ss >> v.T)) v.T = k.T;
     else if (!(ss >> k.batch)) k.batch = 0;
     else if (!(ss >> v.rows)) v.rows = 0;
+    else if (!(ss >> v.pace_ns)) v.pace_ns = 0;
+    if (v.pace_ns < 0)
+      return fail(D3H_ERR_INVALID, "bad plane schedule in %s: %s", path, line.c_str());
     if (k.batch != 0 && k.batch != 1)
       return fail(D3H_ERR_INVALID, "bad batch flag in %s: %s", path, line.c_str());
     if (v.rows != 0 && v.rows != 1)

@@ -62,6 +62,7 @@ struct TuneVal {
   int T;                    // depth of the entry; under a depth-0 key, the tuned depth
   int rows;                 // 1: a 2D entry (stencil2d_kernel); 3D tiles and 2D
                             // entries can share every other parameter
+  int pace_ns = 0;          // depth 1: plane schedule, ns per z iteration (0: none)
 };
 // Size class of a grid: ceil(log2) of each dimension.  Depth 0 keys hold the
 // depth the autotuner chose for the class (d3h_autotune with depth 0).
@@ -95,6 +96,7 @@ struct d3h_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int num_cus = 0;
   int lds_per_block = 0;
+  int rtc_khz = 0;                  // real-time counter rate (0: no plane schedules)
   double *partials = nullptr;       // device scratch for reductions
   double *host_partials = nullptr;  // pinned mirror
   int partial_blocks = 0;
@@ -122,9 +124,14 @@ namespace d3h {
 // ct == cb == 0 (the 2D 5-point operator).
 bool rows_ok(const d3h_field *f, const double coef[7]);
 // Resolve the kernel for a run configuration (tuned / explicit / default);
-// 2D entries are chosen only when `rows` (see rows_ok).
+// 2D entries are chosen only when `rows` (see rows_ok).  *pace_ns: the plane
+// schedule of depth-1 launches (cfg.pace_ns, else the tuned one; 0: none).
 int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
-                  bool rows, const KernelEntry **entry, int *zchunks);
+                  bool rows, const KernelEntry **entry, int *zchunks, int *pace_ns);
+// Geometry::pace of a plane schedule of pace_ns per z iteration on ctx's
+// device (0 when pace_ns <= 0, the entry is fused or the device reports no
+// real-time counter rate).
+int pace_ticks(const d3h_context *ctx, const KernelEntry &e, int pace_ns);
 // Launch geometry of entry `e` on field `f` (2D entries stream the rows of
 // the nz == 1 field in blocks of RPW rows).
 int field_geometry(const KernelEntry &e, const d3h_field *f, int zchunks, Geometry *g,
@@ -133,7 +140,8 @@ int field_geometry(const KernelEntry &e, const d3h_field *f, int zchunks, Geomet
 // sr / seed: stochastic rounding of BF16 storage, step numbers continuing from
 // f->steps_done (the caller advances steps_done).
 int enqueue_steps(d3h_field *f, int count, const double coef[7], const KernelEntry &e, int zchunks,
-                  int sr, unsigned seed, hipStream_t stream, int *launches, unsigned *blocks);
+                  int pace_ns, int sr, unsigned seed, hipStream_t stream, int *launches,
+                  unsigned *blocks);
 // Convert nx*ny*nz cells between device arrays of two dtypes; a row pitch of
 // 0 means dense (pitch = nx).  Narrowing rounds to nearest even (from f64 to
 // bf16/f16 it rounds to f32 first).

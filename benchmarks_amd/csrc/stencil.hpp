@@ -104,6 +104,10 @@ struct Geometry {
   // row count; plane q holds rows [q*RPW, q*RPW + RPW) (the last one clipped).
   // 0 for every other kernel.
   int rows2d;
+  // Plane schedule of depth-1 tiles (0: none): a workgroup starts its z
+  // iteration j (counted from its first) no earlier than its start time plus
+  // j * pace / 256 ticks of the device's real-time counter.
+  int pace;
 };
 
 template <typename C> struct Coef { C ce, cw, cn, cs, ct, cb, cc; };
@@ -456,10 +460,13 @@ struct TileRunner {
   int rowskip;          // bit (s - 1) * RPW + r: stage s need not compute row r (uniform)
   uint32_t sr_lane[G];  // 16-bit storage: Weyl offset of the lane's first x pair per group
   float cwv, cev;       // cw and ce in VGPRs (mul_dpp's second operand)
+  uint64_t t_start;     // depth 1: real-time counter at the workgroup's start
+  int i_first;          // depth 1: first z iteration (the plane schedule's origin)
 
   __device__ __forceinline__ TileRunner(const ST *s_, ST *d_, const Geometry &g, const Coef<C> &k_,
                                         C *x_, int tx, int ty, int tz)
       : src(s_), dst(d_), geo(g), k(k_), xch(x_) {
+    if constexpr (T == 1) t_start = __builtin_amdgcn_s_memrealtime();
     lane = __lane_id();
     wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int xo0 = tx * geo.ox;
@@ -818,11 +825,27 @@ struct TileRunner {
     if constexpr (s < T) stages<PH, s + 1>(i, buf);
   }
 
+  // Depth 1: hold the workgroup to its plane schedule (geo.pace) before it
+  // requests the next plane.  Every workgroup streams its own column of
+  // planes; held to one schedule, the streams of all workgroups stay on
+  // nearby planes instead of drifting apart over the launch, and HBM serves
+  // such compact fronts faster (the autotuner measures whether a schedule
+  // helps and how fast it may run).  A workgroup that is behind never waits,
+  // and the wait is bounded (4096 sleeps of 64 cycles), so a schedule can
+  // only slow a launch down, never stall it.
+  __device__ __forceinline__ void pace(int i) {
+    if (geo.pace > 0) {
+      const uint64_t due = t_start + (((uint64_t)(uint32_t)(i - i_first) * (uint32_t)geo.pace) >> 8);
+      for (int n = 0; n < 4096 && __builtin_amdgcn_s_memrealtime() < due; ++n) __builtin_amdgcn_s_sleep(1);
+    }
+  }
+
   // Iteration i (i == PH mod NS): request plane i + PF, exchange the wave
   // edge rows, run the stages (PF = 0: request plane i + 1 after stage 1).
   // The exchange buffer alternates with i.
   template <int PH>
   __device__ __forceinline__ void iter(int i) {
+    if constexpr (T == 1) pace(i);
     if constexpr (PF > 0) load_plane<modn<NS>(PH + PF)>(i + PF);
     const int buf = (X::kBufs == 2) ? (i & 1) : 0;
     if constexpr (NW > 1) {
@@ -853,6 +876,7 @@ struct TileRunner {
     // NS - 1 planes past its end (their stores are dropped).  The host sizes
     // interior chunks to whole groups.
     const int i0 = zl0 - (PF > 0 ? PF : 1);
+    i_first = i0;
     const int ilast = zo1 - 1 + T;
     for (int i = i0; i <= ilast; i += NS) iters<0>(i);
   }

@@ -245,11 +245,12 @@ class Field:
         return out
 
     def run(self, count, coef, depth=1, arith=N.ARITH_EXACT, tile=-1, zchunks=0, graph=False,
-            rounding=N.ROUND_NEAREST, seed=0):
+            rounding=N.ROUND_NEAREST, seed=0, pace_ns=0):
         """Advance `count` steps.  rounding=ROUND_STOCHASTIC (BF16 fields) rounds
         every stored level stochastically with `seed`; step numbers continue
-        from the last upload."""
-        cfg = N.RunConfig(depth, arith, tile, zchunks, 1 if graph else 0, rounding, seed)
+        from the last upload.  pace_ns: plane schedule of depth-1 launches
+        (0 = the tuned one, -1 = none, > 0 = ns per z iteration)."""
+        cfg = N.RunConfig(depth, arith, tile, zchunks, 1 if graph else 0, rounding, seed, pace_ns)
         t = N.Timing()
         check(lib().d3h_field_run(self.handle, count, _coef_array(coef), ctypes.byref(cfg),
                                   ctypes.byref(t)), "d3h_field_run")

@@ -126,6 +126,12 @@ typedef struct {
   int use_graph;       /* 1: capture the launch sequence in a hipGraph         */
   int rounding;        /* D3H_ROUND_*; STOCHASTIC needs a BF16 or FP16 field   */
   unsigned int seed;   /* seed of stochastic rounding                          */
+  int pace_ns;         /* plane schedule of depth-1 launches, ns per z
+                          iteration: 0 = the tuned one (tuned tile and
+                          z-chunks only), -1 = none, > 0 = this one.  A
+                          schedule only holds workgroups back; results are
+                          the same with any schedule.  No counterpart in the
+                          reference.                                         */
 } d3h_run_config;
 
 typedef struct {
@@ -138,6 +144,7 @@ typedef struct {
   int tile;               /* kernel/tile index used                           */
   int zchunks;            /* z-chunks used                                    */
   int blocks;             /* workgroups per launch                            */
+  int pace_ns;            /* plane schedule used (0: none)                    */
 } d3h_timing;
 
 void d3h_run_config_default(d3h_run_config *cfg);
