@@ -53,6 +53,23 @@ def test_every_plain_3d_tile_with_schedules_matches_oracle(ctx, shape):
                 assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), (t, pace, zc)
 
 
+def test_every_f64_plain_tile_with_schedules_matches_oracle(ctx):
+    shape = (200, 37, 41)
+    k = d3.constants_f64(*shape)
+    f0 = d3.initialize_f64(*shape, k)
+    want = oracle.run_f64(f0, k[8:15], 5)
+    tiles = [t for t in d3.tiles() if t["dtype"] == d3.F64 and t["temporal_depth"] == 1
+             and t["arith"] == d3.ARITH_EXACT and not t["row_stream"]]
+    assert tiles
+    for t in tiles:
+        for pace in PACES:
+            print("F64 tile", t["index"], "pace", pace, flush=True)
+            out, tm = run(ctx, f0, d3.coefficients(k), 5, dtype=d3.F64, tile=t["index"], zchunks=3,
+                          pace_ns=pace)
+            assert tm["pace_ns"] == max(pace, 0), tm
+            assert np.array_equal(out.view(np.uint64), want.view(np.uint64)), (t, pace)
+
+
 def test_fused_depths_ignore_schedules(ctx):
     shape = (130, 20, 33)
     k, f0 = noisy_f32(shape)
