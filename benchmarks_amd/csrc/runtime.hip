@@ -1188,7 +1188,8 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
     }
   }
   // Depth 1: plane schedules (Geometry::pace) for the three fastest
-  // configurations, at 0.86-0.99 of each one's measured time per z iteration.
+  // configurations, at 0.86-0.99 of each one's measured time per z iteration
+  // (launch time / (iterations x rounds of resident workgroups)).
   // A schedule that only the fastest workgroups reach keeps the streams of all
   // workgroups on nearby planes; its optimum is narrow (on the 512^3 FP32 step
   // about 0.93 of the unscheduled time per iteration for the 4-wave tile: 4-5 %
@@ -1204,7 +1205,10 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
       const KernelEntry &e = reg[c.entry];
       const int halo = e.rows ? (T + e.RPW - 1) / e.RPW : T;
       const int iters = c.g.oz + T + halo + std::max(e.pf, 1);
-      const double ns_iter = c.best * 1e6 / iters;
+      // a launch with more workgroups than fit at once runs them in rounds
+      const long long slots = (long long)std::max(occupancy_of(e), 1) * std::max(ctx->num_cus, 1);
+      const long long rounds = std::max(1LL, ((long long)c.blocks + slots - 1) / slots);
+      const double ns_iter = c.best * 1e6 / ((double)iters * rounds);
       for (int pct = 86; pct <= 99; ++pct) {
         Cand p = c;
         p.best = 1e300;
