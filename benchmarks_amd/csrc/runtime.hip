@@ -196,6 +196,10 @@ int tuned_depth(d3h_context *ctx, int dtype, int fma, int nx, int ny, int nz, in
   return it != ctx->tuned.end() ? it->second.T : dflt;
 }
 
+// A schedule slower than this per z iteration is refused: a plane takes
+// microseconds, and a wait ends at its bound long before.
+constexpr int kMaxPaceNs = 1000000;
+
 int pace_ticks(const d3h_context *ctx, const KernelEntry &e, int pace_ns) {
   if (pace_ns <= 0 || e.T != 1 || ctx->rtc_khz <= 0) return 0;
   const long long t = (long long)pace_ns * ctx->rtc_khz * 256 / 1000000;
@@ -205,6 +209,9 @@ int pace_ticks(const d3h_context *ctx, const KernelEntry &e, int pace_ns) {
 int resolve_entry(d3h_context *ctx, int nx, int ny, int nz, int dtype, const d3h_run_config &cfg,
                   bool rows, const KernelEntry **entry, int *zchunks, int *pace_ns) {
   // an explicit schedule (> 0) or none (< 0) overrides the tuned one
+  if (cfg.pace_ns > kMaxPaceNs)
+    return fail(D3H_ERR_INVALID, "plane schedule of %d ns per iteration (at most %d)", cfg.pace_ns,
+                kMaxPaceNs);
   *pace_ns = cfg.pace_ns > 0 ? cfg.pace_ns : 0;
   int T = cfg.temporal_depth;
   const int fma = cfg.arith == D3H_ARITH_FMA ? 1 : 0;
@@ -1348,7 +1355,7 @@ int d3h_autotune_load(d3h_context *ctx, const char *path) {
     else if (!(ss >> k.batch)) k.batch = 0;
     else if (!(ss >> v.rows)) v.rows = 0;
     else if (!(ss >> v.pace_ns)) v.pace_ns = 0;
-    if (v.pace_ns < 0)
+    if (v.pace_ns < 0 || v.pace_ns > kMaxPaceNs)
       return fail(D3H_ERR_INVALID, "bad plane schedule in %s: %s", path, line.c_str());
     if (k.batch != 0 && k.batch != 1)
       return fail(D3H_ERR_INVALID, "bad batch flag in %s: %s", path, line.c_str());

@@ -128,8 +128,8 @@ typedef struct {
   unsigned int seed;   /* seed of stochastic rounding                          */
   int pace_ns;         /* plane schedule of depth-1 launches, ns per z
                           iteration: 0 = the tuned one (tuned tile and
-                          z-chunks only), -1 = none, > 0 = this one.  A
-                          schedule only holds workgroups back; results are
+                          z-chunks only), -1 = none, 1..1000000 = this one.
+                          A schedule only holds workgroups back; results are
                           the same with any schedule.  No counterpart in the
                           reference.                                         */
 } d3h_run_config;

@@ -70,6 +70,20 @@ def test_every_f64_plain_tile_with_schedules_matches_oracle(ctx):
             assert np.array_equal(out.view(np.uint64), want.view(np.uint64)), (t, pace)
 
 
+def test_schedules_beyond_a_millisecond_are_refused(ctx, tmp_path):
+    with d3.Field(ctx, 16, 16, 16) as f:
+        f.upload(np.zeros((16, 16, 16), np.float32))
+        with pytest.raises(d3.D3HError):
+            f.run(1, [0.1] * 6 + [0.4], pace_ns=1000001)
+        tm = f.run(1, [0.1] * 6 + [0.4], pace_ns=1000000)
+        assert tm["pace_ns"] in (0, 1000000)
+    bad = tmp_path / "slow.txt"
+    # dtype depth fma log2nx log2ny log2nz vec G RPW NW pf zchunks ms depth batch rows pace_ns
+    bad.write_text("0 1 0 4 4 4 4 2 1 4 1 0 0.1 1 0 0 1000001\n")
+    with pytest.raises(d3.D3HError):
+        ctx.load_tuning(str(bad))
+
+
 def test_fused_depths_ignore_schedules(ctx):
     shape = (130, 20, 33)
     k, f0 = noisy_f32(shape)
