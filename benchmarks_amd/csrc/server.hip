@@ -5,8 +5,12 @@
 //     admit   : pop queued requests while a slot is free and the request's
 //               footprint (two pitched buffers + conversion staging) fits
 //               in the server's device pool (first-fit, coalescing free list)
-//     upload  : H2D (+ dtype conversion) on the copy stream; the compute
-//               stream waits on it before the next sweep
+//     upload  : H2D (+ dtype conversion) on the copy stream, marked by an
+//               event of the request's own; a request joins the sweeps once
+//               its event has completed, so the requests already on the
+//               device do not wait for a newcomer's transfer (unless more
+//               cells are uploading than are on the device: then the sweep
+//               waits for the uploads)
 //     sweep   : every active request advances min(remaining, sweep_steps)
 //               steps; one launch per kernel class covers all of them (the
 //               batched sweep planner), so requests that arrived while others
@@ -97,6 +101,8 @@ struct Request {
   int pitch = 0;
   int cur = 0;
   int remaining = 0;
+  hipEvent_t up_ev = nullptr;  // marks the end of the upload (scheduler)
+  bool uploaded = false;       // up_ev has completed: the request can sweep
   bool freed = false;     // pool memory returned (scheduler)
   bool released = false;  // record released by d3h_server_wait before `freed`
 };
@@ -112,7 +118,7 @@ struct d3h_server {
   hipStream_t compute = nullptr, copy = nullptr;
   std::vector<hipStream_t> extra;
   std::vector<hipEvent_t> fork_events;
-  hipEvent_t upload_ev = nullptr;
+  std::vector<hipEvent_t> ev_free;  // upload events not in use (scheduler)
   hipEvent_t sweep_ev[2] = {nullptr, nullptr};
   bool sweep_pending[2] = {false, false};
   void *dev_tables[2] = {nullptr, nullptr};
@@ -178,6 +184,30 @@ int upload(d3h_server *s, Request *r) {
   char *stage = static_cast<char *>(s->pool) + r->stage_off;
   D3H_TRY(copy_in(r->in, r->in_dt, field_ptr(s, r, 0), p.dtype, r->pitch, p.nx, p.ny, p.nz, stage, s->copy));
   r->cur = 0;
+  if (s->ev_free.empty()) {
+    hipEvent_t ev = nullptr;
+    D3H_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    s->ev_free.push_back(ev);
+  }
+  r->up_ev = s->ev_free.back();
+  s->ev_free.pop_back();
+  r->uploaded = false;
+  D3H_HIP(hipEventRecord(r->up_ev, s->copy));
+  return D3H_OK;
+}
+
+// Has r's upload completed?  Its event then goes back to the free list, and
+// the compute stream is ordered after it (already satisfied: no wait).
+int poll_upload(d3h_server *s, Request *r) {
+  if (r->uploaded) return D3H_OK;
+  const hipError_t e = hipEventQuery(r->up_ev);
+  if (e == hipErrorNotReady) return D3H_OK;
+  if (e != hipSuccess) return fail(D3H_ERR_HIP, "upload of request %lld: %s", (long long)r->id,
+                                   hipGetErrorString(e));
+  D3H_HIP(hipStreamWaitEvent(s->compute, r->up_ev, 0));
+  s->ev_free.push_back(r->up_ev);
+  r->up_ev = nullptr;
+  r->uploaded = true;
   return D3H_OK;
 }
 
@@ -229,15 +259,39 @@ int round_once(d3h_server *s, bool *did_work) {
       D3H_TRY(upload(s, r));
       r->remaining = r->p.steps;
     }
-    D3H_HIP(hipEventRecord(s->upload_ev, s->copy));
-    D3H_HIP(hipStreamWaitEvent(s->compute, s->upload_ev, 0));
     s->active.insert(s->active.end(), admitted.begin(), admitted.end());
     *did_work = true;
   }
-  std::vector<Request *> members, finishing;
+  std::vector<Request *> members, pending, finishing;
+  double ready_cells = 0, pending_cells = 0;
   for (Request *r : s->active) {
-    if (r->remaining > 0) members.push_back(r);
-    else finishing.push_back(r);  // zero-step requests
+    if (r->remaining == 0) {
+      finishing.push_back(r);  // zero-step requests (copied back after their upload)
+      continue;
+    }
+    D3H_TRY(poll_upload(s, r));
+    const double cells = (double)r->p.nx * r->p.ny * r->p.nz;
+    if (r->uploaded) {
+      members.push_back(r);
+      ready_cells += cells;
+    } else {
+      pending.push_back(r);
+      pending_cells += cells;
+    }
+  }
+  // A burst of arrivals (more cells still uploading than on the device): the
+  // sweep waits for those uploads instead of running the few ready requests
+  // alone, whose small sweeps would leave the device mostly idle and finish
+  // the burst later.  A single newcomer among running requests joins once
+  // its upload has completed, without holding them up.
+  if (!pending.empty() && ready_cells < pending_cells) {
+    for (Request *r : pending) {
+      D3H_HIP(hipStreamWaitEvent(s->compute, r->up_ev, 0));
+      s->ev_free.push_back(r->up_ev);  // a later record does not affect this wait
+      r->up_ev = nullptr;
+      r->uploaded = true;
+      members.push_back(r);
+    }
   }
   if (!members.empty()) {
     const int slot = (int)(s->sweeps & 1);
@@ -279,7 +333,13 @@ int round_once(d3h_server *s, bool *did_work) {
     *did_work = true;
   }
   if (!finishing.empty()) {
-    for (Request *r : finishing) D3H_TRY(download(s, r));
+    for (Request *r : finishing) {
+      if (r->up_ev) {  // a zero-step request: its download follows the upload on the copy stream
+        s->ev_free.push_back(r->up_ev);
+        r->up_ev = nullptr;
+      }
+      D3H_TRY(download(s, r));
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     for (Request *r : finishing) {
       r->state = r->state.load() == kDone ? (int)kDone : (int)kDraining;
@@ -320,9 +380,11 @@ void scheduler(d3h_server *s) {
     }
     if (s->stop && idle && s->draining.empty()) return;
     if (!work) {
-      if (!s->draining.empty() || !s->queue.empty())
+      if (!s->active.empty())  // uploads in flight
+        s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+      else if (!s->draining.empty() || !s->queue.empty())
         s->cv_work.wait_for(lk, std::chrono::microseconds(100));
-      else if (s->active.empty())
+      else
         s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
     }
   }
@@ -365,7 +427,6 @@ int d3h_server_create(d3h_context *ctx, const d3h_server_config *cfg_in, d3h_ser
     ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
     if (ok) s->fork_events.push_back(ev);
   }
-  ok = ok && hipEventCreateWithFlags(&s->upload_ev, hipEventDisableTiming) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&s->sweep_ev[0], hipEventDisableTiming) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&s->sweep_ev[1], hipEventDisableTiming) == hipSuccess;
   if (!ok) {
@@ -394,7 +455,9 @@ int d3h_server_destroy(d3h_server *s) {
   if (s->copy) (void)hipStreamSynchronize(s->copy);
   for (auto st : s->extra) (void)hipStreamDestroy(st);
   for (auto ev : s->fork_events) (void)hipEventDestroy(ev);
-  if (s->upload_ev) (void)hipEventDestroy(s->upload_ev);
+  for (auto ev : s->ev_free) (void)hipEventDestroy(ev);
+  for (Request *r : s->active)
+    if (r->up_ev) (void)hipEventDestroy(r->up_ev);
   for (int i = 0; i < 2; ++i) {
     if (s->sweep_ev[i]) (void)hipEventDestroy(s->sweep_ev[i]);
     if (s->dev_tables[i]) (void)hipFree(s->dev_tables[i]);
