@@ -1145,8 +1145,14 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
     if (field_geometry(e, f, 0, &g, &nb) != D3H_OK) continue;
     // planes (rows, row blocks) streamed
     const int nstream = e.rows ? (f->ny + e.RPW - 1) / e.RPW : f->nz;
-    // few long streams (1-4 chunks) up to several waves of workgroups
+    // few long streams (1-4 chunks) up to several waves of workgroups; 2D
+    // fields also try a quarter and an eighth of the default count, where
+    // plane schedules gained most (16384^2: 2-row blocks in 16-23 chunks)
     std::vector<int> zcs = {1, 2, 3, 4, g.nzc, g.nzc * 2, g.nzc * 4, std::max(1, g.nzc / 2)};
+    if (only_2d) {
+      zcs.push_back(std::max(1, g.nzc / 4));
+      zcs.push_back(std::max(1, g.nzc / 8));
+    }
     std::sort(zcs.begin(), zcs.end());
     zcs.erase(std::unique(zcs.begin(), zcs.end()), zcs.end());
     int last_nzc = -1;
@@ -1201,7 +1207,7 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
       *bz = c.g.nzc;
     }
   }
-  // Depth 1: plane schedules (Geometry::pace) for the three fastest
+  // Depth 1: plane schedules (Geometry::pace) for the six fastest
   // configurations, at 0.86-0.99 of each one's measured time per z iteration
   // (launch time / (iterations x rounds of resident workgroups)).
   // A schedule that only the fastest workgroups reach keeps the streams of all
@@ -1213,7 +1219,7 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
   if (T == 1 && ctx->rtc_khz > 0 && !cands.empty()) {
     std::vector<Cand> top = cands;
     std::sort(top.begin(), top.end(), [](const Cand &a, const Cand &b) { return a.best < b.best; });
-    if (top.size() > 3) top.resize(3);
+    if (top.size() > 6) top.resize(6);
     std::vector<Cand> paced;
     for (const Cand &c : top) {
       const KernelEntry &e = reg[c.entry];
