@@ -166,3 +166,23 @@ def test_tuned_schedule_travels_through_the_tune_table(tmp_path):
     with d3.Context(0) as c4:
         with pytest.raises(d3.D3HError):
             c4.load_tuning(str(bad))
+
+
+def test_full_size_tuned_schedule_equals_unscheduled():
+    """512^3 FP32 (the benchmark's grid), the autotuner's tile, z-chunks and
+    schedule against the same tile unscheduled: bitwise equal after 20 steps
+    (device-side comparison)."""
+    n = 512
+    k = d3.constants_f32(n, n, n)
+    f0 = d3.initialize_f32(n, n, n, k)
+    coef = d3.coefficients(k)
+    with d3.Context(0) as c:
+        tuned = c.autotune(n, n, n, d3.F32, 1)
+        with d3.Field(c, n, n, n) as a, d3.Field(c, n, n, n) as b:
+            a.upload(f0)
+            b.upload(f0)
+            ta = a.run(20, coef)
+            tb = b.run(20, coef, tile=ta["tile"], zchunks=ta["zchunks"], pace_ns=-1)
+            print("tuned", tuned, "scheduled", ta, "unscheduled", tb, flush=True)
+            assert ta["tile"] == tuned["tile"] and tb["pace_ns"] == 0
+            assert a.compare(b)["max_abs"] == 0.0
