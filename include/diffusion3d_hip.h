@@ -240,11 +240,14 @@ int d3h_tile_info(int index, d3h_tile_desc *out);
  * the given shape, remember the fastest for that size class; the table can be
  * saved to / loaded from a text file.  best_tile / best_zchunks may be NULL.
  * temporal_depth 0 searches depths 1, 2 and 4 (and 8 for nz == 1 fields,
- * whose 2D kernels gain from it; 3D depth-8 tiles measured slower than plain
- * steps) and also remembers the fastest depth for the class, used by runs, batches and servers given depth
- * 0; best_tile then names the chosen depth's tile (d3h_tile_info).  Every
- * candidate is timed twice, interleaved with the others, and keeps its
- * faster time; BF16/FP16 storage is timed with stochastic rounding. */
+ * whose 2D kernels gain from it; 3D depth-8 tiles are far slower than depth
+ * 4) and also remembers the fastest depth for the class, used by runs,
+ * batches and servers given depth 0; best_tile then names the chosen depth's
+ * tile (d3h_tile_info).  Candidates are timed in two interleaved passes (the
+ * second only for those within 8 % of the first pass's fastest), each keeping
+ * its faster time; at depth 1 the six fastest also get plane schedules
+ * (d3h_run_config.pace_ns), and the tuned schedule is remembered with the
+ * tile.  BF16/FP16 storage is timed with stochastic rounding. */
 int d3h_autotune(d3h_context *ctx, int nx, int ny, int nz, int dtype,
                  int temporal_depth, int arith, int *best_tile, int *best_zchunks,
                  double *best_ms_per_step);
