@@ -24,6 +24,34 @@ def test_library_exports_every_header_symbol():
     assert sorted(N.SIGNATURES) == names
 
 
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every struct the Python binding mirrors has the header's size and field
+    offsets (the header compiled by gcc as C)."""
+    mirrors = {"d3h_run_config": N.RunConfig, "d3h_timing": N.Timing, "d3h_tile_desc": N.TileDesc,
+               "d3h_problem": N.Problem, "d3h_server_config": N.ServerConfig}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "diffusion3d_hip.h"', "int main(void) {"]
+    for c, py in mirrors.items():
+        lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (c, c))
+        for name, _ in py._fields_:
+            lines.append('  printf("%s %s %%zu\\n", offsetof(%s, %s));' % (c, name, c, name))
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            key, value = line.rsplit(" ", 1)
+            got[key] = int(value)
+    for c, py in mirrors.items():
+        assert got["%s size" % c] == ctypes.sizeof(py), c
+        for name, _ in py._fields_:
+            assert got["%s %s" % (c, name)] == getattr(py, name).offset, (c, name)
+
+
 def test_host_constants_match_reference(golden):
     meta, _ = golden
     for key, bits in meta["constants"].items():
