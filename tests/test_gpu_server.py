@@ -83,3 +83,34 @@ def test_server_admission_by_memory_budget(ctx):
         assert srv.stats()["rejected"] == 1
     for i, (p, f0) in enumerate(reqs):
         assert np.array_equal(outs[i].view(np.uint32), expected(p, f0).view(np.uint32)), i
+
+
+def test_newcomers_and_zero_step_requests_join_running_sweeps(ctx):
+    """A large request is running when small ones, zero-step ones and a second
+    large one arrive (each joins the sweeps once its own upload has completed,
+    or a burst waits for its uploads): every result equals the oracle run of
+    that request alone, bitwise."""
+    rng = np.random.default_rng(77)
+    big = []
+    for i, n in enumerate((150, 120)):
+        coef, _ = d3.stable_coefficients(n, n, n)
+        k = d3.constants_f32(n, n, n)
+        f0 = d3.add_noise_f32(d3.initialize_f32(n, n, n, k), 500 + i, 1e-3)
+        big.append(({"nx": n, "ny": n, "nz": n, "dtype": d3.F32, "steps": 60, "coef": coef}, f0))
+    small = [make_request(rng, i, lo=10, hi=60, max_steps=25) for i in range(12)]
+    for j in (3, 7):  # zero-step requests: the result is the input
+        small[j] = (dict(small[j][0], steps=0), small[j][1])
+    reqs = [big[0]] + small[:6] + [big[1]] + small[6:]
+    outs = [np.empty(f0.shape, np.float32) for _, f0 in reqs]
+    with d3.Server(ctx, hbm_budget_bytes=1 << 30, sweep_steps=4) as srv:
+        ids = [srv.submit(reqs[0][0], reqs[0][1], d3.F32, outs[0])]
+        time.sleep(0.002)
+        for i in range(1, len(reqs)):
+            ids.append(srv.submit(reqs[i][0], reqs[i][1], d3.F32, outs[i]))
+            if i % 3 == 0:
+                time.sleep(0.001)
+        for r in ids:
+            srv.wait(r)
+    for i, ((p, f0), out) in enumerate(zip(reqs, outs)):
+        want = expected(p, f0)
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), (i, p["steps"])
