@@ -1232,7 +1232,9 @@ static int tune_depth(d3h_context *ctx, d3h_field *f, int T, int fma, const doub
       for (int pct = 86; pct <= 99; ++pct) {
         Cand p = c;
         p.best = 1e300;
-        p.pace_ns = (int)(ns_iter * pct / 100.0);
+        const double ns = ns_iter * pct / 100.0;
+        if (ns > kMaxPaceNs) break;  // a tune table refuses such schedules
+        p.pace_ns = (int)ns;
         p.g.pace = pace_ticks(ctx, e, p.pace_ns);
         if (p.g.pace > 0) paced.push_back(p);
       }
