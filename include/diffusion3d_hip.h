@@ -343,7 +343,10 @@ int d3h_batch_autotune(d3h_batch *b, int arith, int *classes, double *ms_before,
  * A request server over one device: requests (grid + steps + initial field)
  * are admitted as device memory and slots allow, kept resident, batched with
  * whatever else is running (continuous batching at sweep boundaries), and
- * copied back when done.  Submission and polling are thread-safe. */
+ * copied back when done.  An admitted request joins the sweeps once its own
+ * upload has completed, so running requests are not held up by it; when more
+ * cells are still uploading than are ready, the next sweep waits for those
+ * uploads instead.  Submission and polling are thread-safe. */
 typedef struct d3h_server d3h_server;
 typedef struct {
   size_t hbm_budget_bytes;  /* device bytes the server may hold; 0 = 90% of free */
