@@ -10,8 +10,9 @@ exactly --steps steps (the library records HIP events on its own stream around
 the launches and synchronises on the last one, so the region is bracketed by
 device synchronisation on both sides).  value = Gcell-updates/s from the host
 wall clock of that call.  The steps are launched directly (--graph 0): on the
-512^3 grid a hipGraph replay measured 0.2-0.27 ms of fixed cost per call and
-about 1 % more per launch than direct launches, whose fixed cost is ~13 us.
+512^3 grid a hipGraph replay takes the same time per launch as direct launches
+but adds a fixed cost per call (0.2-0.27 ms inside the events at 20 steps,
+about 0.7 ms of wall clock at 400), against ~20 us for direct launches.
 
 roofline: algorithmic bytes per launch (2 x storage bytes x cells, one read
 and one write per cell, as the reference's GetThroughput) / average launch
